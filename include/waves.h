@@ -1,0 +1,119 @@
+/*
+ * waves_hip -- C-ABI of the MI355X (gfx950) free-surface solver step.
+ *
+ * Drop-in for the GPU side of the reference's solver runtime: the Vulkan
+ * compute dispatch of water/work_scheduler.gd, the kernels under
+ * water/shaders/compute, and the multigrid hierarchy of
+ * water/pressure_multigrid.gd and water/pressure_hierarchy_state.gd.  The host
+ * keeps the reference's step / reset / readback surface
+ * (water/pressure_gpu.gd:40-59, 133-137) and its staggered MAC buffer layout.
+ *
+ * Conventions: plain pointers and sizes, no C++ or framework types; every
+ * function returns 0 on success and a negative WV_E* code on a host or HIP
+ * error (never throws); wv_last_error() returns the message.  Device failure
+ * codes stay in the control vector c[7] exactly as in the reference
+ * (pressure_pass.glslinc, coupled_motion.glslinc, coastal_pass.glslinc).
+ * One solver handle owns its device memory and one HIP stream.  request_*
+ * may be called from any thread; all other calls from one thread per handle.
+ */
+#ifndef WAVES_HIP_H
+#define WAVES_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wv_solver wv_solver;
+
+/* Settings: the 32-float parameter block of water/pressure_gpu.gd:68 +
+ * water/coupled_gpu.gd:18-21 + water/coastal_gpu.gd:23-24 (index 3 = op and
+ * 15 = quota are ignored), plus 8 option floats: right absorption
+ * (water/coastal_gpu.gd:12), MG smoothing / lag / aggregate_from / maintenance
+ * mode (water/pressure_gpu.gd:42-43), PCG iterations per batch
+ * (water/coupled_gpu.gd:14), alongshore perturbation amplitude and seed. */
+typedef struct wv_settings {
+  float pc[32];
+  float opts[8];
+} wv_settings;
+
+enum {
+  WV_OK = 0,
+  WV_EINVAL = -1,   /* bad argument or size */
+  WV_EHIP = -2,     /* a HIP runtime call failed */
+  WV_ENOMEM = -3,   /* device allocation failed */
+  WV_ENODEV = -4,   /* no gfx950 device */
+  WV_EPLAN = -5     /* multigrid planning failed (reference host failure 97) */
+};
+
+/* Field ids for wv_read_field / wv_write_field (layouts of
+ * pressure_core.glslinc:2-11 and coupled_motion.glslinc:1-14). */
+enum {
+  WV_FIELD_BANK_A = 0,        /* [F N][U][V][W][P N] float */
+  WV_FIELD_BANK_B = 1,
+  WV_FIELD_WORK = 2,          /* 14 N float */
+  WV_FIELD_GEO = 3,           /* 2 N float (phi, kind) */
+  WV_FIELD_CONTROL = 4,       /* 96 float, the reference's c[] */
+  WV_FIELD_PLANE = 5,         /* 4 N float, reconstructed local PLIC */
+  WV_FIELD_MOTION = 6,        /* 4 N + 12 faces float */
+  WV_FIELD_IMG_PLANES = 7,    /* 4 N float, exported accepted planes */
+  WV_FIELD_IMG_KINDS = 8,     /* N float, exported kinds 0/1/2 */
+  WV_FIELD_HISTORY = 9,       /* capacity*4 float4 step history */
+  WV_FIELD_COASTAL_HISTORY = 10, /* capacity*5 float4 */
+  WV_FIELD_PROBE = 11,        /* 8 float4 analytic probes */
+  WV_FIELD_PARTIAL = 12,      /* ceil(N/128) float4 */
+  WV_FIELD_MG_VECTORS = 13,   /* 8 floats per MG node, per level */
+  WV_FIELD_SNAPSHOT = 14,     /* 9 N float, preconditioner snapshot */
+  WV_FIELD_AUDIT = 15,        /* hierarchy audit state (uint32) */
+  WV_FIELD_MG_ROWS = 16,      /* coarse operator rows, k-major, levels >= 1 */
+  WV_FIELD_MG_LINV = 17       /* last level: dense L^-1, row-major */
+};
+
+/* _init(settings) + start() + _initialize (water/pressure_gpu.gd:40-47, 82-127):
+ * plans the MG hierarchy (water/pressure_multigrid.gd:33-134), allocates every
+ * device buffer, clears them. Does not run the initial commands. */
+int wv_create(const wv_settings *settings, wv_solver **out);
+/* close() / _free() (water/pressure_gpu.gd:55-59, 138-149) */
+int wv_destroy(wv_solver *s);
+/* initial command list (water/coupled_gpu.gd:34-37, water/coastal_gpu.gd:26-27);
+ * a reset of the reference is close + new + start (water/lab_runner.gd:15-31). */
+int wv_reset(wv_solver *s);
+/* request(steps) / request_time(seconds) (water/pressure_gpu.gd:48-54): the
+ * quota the device's trial start checks (pressure_pass.glslinc:20). Thread-safe. */
+int wv_request_steps(wv_solver *s, int64_t steps);
+int wv_request_time(wv_solver *s, double seconds);
+/* frame() (water/work_scheduler.gd:41-87): enqueue `batches` whole batches of
+ * the coupled step program (water/coupled_gpu.gd:38-56) with the current quota,
+ * followed by one asynchronous 384-byte control readback. Non-blocking. */
+int wv_advance(wv_solver *s, int batches);
+/* buffer_get_data_async(control) callback (water/work_scheduler.gd:80-87):
+ * copies the newest completed readback into out[96]; *fresh = 1 if it is newer
+ * than the last poll. With wait != 0 blocks until all enqueued work is done. */
+int wv_poll_control(wv_solver *s, float *out96, int *fresh, int wait);
+/* parity / render export I/O (no reference counterpart: the reference's
+ * buffers are GPU-only; the planes/kinds export replaces the Texture3DRD
+ * upload of water/pressure_gpu.gd:106-115). Synchronous. */
+int64_t wv_field_bytes(wv_solver *s, int field);
+int wv_read_field(wv_solver *s, int field, void *dst, int64_t bytes);
+int wv_write_field(wv_solver *s, int field, const void *src, int64_t bytes);
+/* one reference op (water/pressure_gpu.gd:69-81 _dispatch without the MG hook),
+ * for the parity harness; ops whose finaliser is fused into the previous op's
+ * kernel are no-ops. Enqueued, not waited for. */
+int wv_op(wv_solver *s, int op);
+/* multigrid.record(list, build, initial) (water/pressure_multigrid.gd:155-176) */
+int wv_mg_record(wv_solver *s, int build, int initial);
+/* wait for everything enqueued on the handle's stream */
+int wv_synchronize(wv_solver *s);
+const char *wv_last_error(wv_solver *s);
+/* MG level descriptors: 16 ints per level, the reference's `meta` layout
+ * (water/pressure_multigrid.gd:54, 61-68). Returns the level count. */
+int wv_levels(wv_solver *s, int *meta16, int max_levels);
+/* Average device duration [us] of one launch of a hot kernel, timed with HIP
+ * events on the solver's stream over `reps` launches on the current state
+ * (kernel: 0 = level-0 Chebyshev sweep, 1 = fine masked 7-point SpMV,
+ * 2 = level-0 residual). Does not change the solver state. */
+int wv_time_kernel(wv_solver *s, int kernel, int reps, float *avg_us);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+// PLIC reconstruction, interface geometry and coastal target on the device.
+// Restates water/shaders/compute/plic_math.glslinc, pressure_geometry.glslinc,
+// height_normal.glslinc and coastal_target.glslinc (same operation order).
+#pragma once
+#include "wv_device.h"
+
+// plic_math.glslinc:3-23: unit-cube volume under n.x <= alpha
+__device__ __forceinline__ float cube_volume(F3 n, float alpha) {
+  alpha -= gmin(n.x, 0.0f) + gmin(n.y, 0.0f) + gmin(n.z, 0.0f);
+  float x = fabsf(n.x), y = fabsf(n.y), z = fabsf(n.z);
+  float total = x + y + z;
+  if (alpha <= 0.0f) return 0.0f;
+  if (alpha >= total) return 1.0f;
+  float t;
+  if (x > y) { t = x; x = y; y = t; }
+  if (y > z) { t = y; y = z; z = t; }
+  if (x > y) { t = x; x = y; y = t; }
+  bool complement = alpha > 0.5f * total;
+  t = complement ? total - alpha : alpha;
+  float result;
+  if (x < 1e-6f * total) {
+    t -= x * 0.5f;
+    if (y < 1e-6f * total) result = (t - y * 0.5f) / z;
+    else if (t < y) result = t * t / (2 * y * z);
+    else result = (t - 0.5f * y) / z;
+  } else if (t < x) result = t * t * t / (6 * x * y * z);
+  else if (t < y) result = (3 * t * t - 3 * t * x + x * x) / (6 * y * z);
+  else if (x + y < z && t >= x + y) result = (t - 0.5f * (x + y)) / z;
+  else {
+    float ty = gmax(t - y, 0.0f), tz = gmax(t - z, 0.0f);
+    float numerator = x * (3 * t * t - 3 * t * x + x * x) - powf(ty, 3.0f) - powf(tz, 3.0f);
+    result = numerator / (6 * x * y * z);
+  }
+  return complement ? 1.0f - result : result;
+}
+// plic_math.glslinc:24-37: 25-step bisection for the plane constant
+__device__ __forceinline__ float inverse_volume(F3 n, float fraction) {
+  float lo = gmin(n.x, 0.0f) + gmin(n.y, 0.0f) + gmin(n.z, 0.0f);
+  float hi = gmax(n.x, 0.0f) + gmax(n.y, 0.0f) + gmax(n.z, 0.0f);
+  for (int it = 0; it < 25; it++) { float mid = 0.5f * (lo + hi); if (cube_volume(n, mid) < fraction) lo = mid; else hi = mid; }
+  float alpha = 0.5f * (lo + hi), ctr = dot3(n, mkf(.5f, .5f, .5f));
+  if (fraction == .5f) return ctr;
+  if (fraction < .5f) return gmin(alpha, ctr);
+  if (alpha <= ctr) alpha = ctr + 5.96046448e-8f * dot3(mkf(fabsf(n.x), fabsf(n.y), fabsf(n.z)), mkf(1, 1, 1));
+  return alpha;
+}
+
+// ---- coastal_target.glslinc ----
+__device__ __forceinline__ float smooth_ramp(float q) { q = gclamp(q, 0, 1); return q * q * q * (10 + q * (-15 + 6 * q)); }
+__device__ __forceinline__ float ramped_amplitude(const Ctx& s, float time) {
+  return s.wave[0] * smooth_ramp(time * s.wave[2] / (6.28318530718f * s.forc[3]));
+}
+__device__ __forceinline__ void coastal_surface(const Ctx& s, float x, float time, float& eta_out, float& slope_out) {
+  float a = ramped_amplitude(s, time), k = s.wave[1], w = s.wave[2], h = s.wave[3];
+  float sigma = tanhf(k * h), second = k * a * a * (3 - sigma * sigma) / (4 * sigma * sigma * sigma);
+  if (s.stokes2 < .5f) second = 0;
+  float phase = k * (x - s.forc[0]) - w * time;
+  eta_out = a * cosf(phase) + second * cosf(2 * phase);
+  slope_out = -k * a * sinf(phase) - 2 * k * second * sinf(2 * phase);
+}
+__device__ __forceinline__ F3 coastal_velocity(const Ctx& s, F3 world, float time) {
+  float a = ramped_amplitude(s, time), k = s.wave[1], w = s.wave[2], h = s.wave[3];
+  float sh = sinhf(k * h), phase = k * (world.x - s.forc[0]) - w * time, y = world.y + h;
+  float first = a * w / sh, second = s.stokes2 > .5f ? .75f * a * a * w * k / powf(sh, 4.0f) : 0;
+  return mkf(first * coshf(k * y) * cosf(phase) + second * coshf(2 * k * y) * cosf(2 * phase),
+             first * sinhf(k * y) * sinf(phase) + second * sinhf(2 * k * y) * sinf(2 * phase), 0);
+}
+__device__ __forceinline__ float coastal_velocity_axis(const Ctx& s, F3 world, float time, int axis) {
+  if (axis == 2) return 0.0f;
+  F3 v = coastal_velocity(s, world, time); return axis == 0 ? v.x : v.y;
+}
+__device__ __forceinline__ F3 coastal_face_position(const Ctx& s, I3 q, int axis) { F3 p = center(s, q); comp(p, axis) -= .5f * s.h[axis]; return p; }
+__device__ __forceinline__ float4 coastal_plane(const Ctx& s, I3 q, float time) {
+  float se, ss; coastal_surface(s, center(s, q).x, time, se, ss);
+  F3 normal = mkf(-ss * s.h[0], s.h[1], 0);
+  return make_float4(normal.x, normal.y, normal.z, dot3(normal, mkf(.5f, .5f, .5f)) + se - center(s, q).y);
+}
+__device__ __forceinline__ float coastal_fraction(const Ctx& s, I3 q, float time) {
+  float4 eq = coastal_plane(s, q, time);
+  return gclamp(cube_volume(mkf(eq.x, eq.y, eq.z), eq.w), 0, 1);
+}
+__device__ __forceinline__ float coastal_rate(const Ctx& s, float x) {
+  float left = 1 - smooth_ramp((x - s.forc[0]) / s.forc[1]);
+  float right = s.shape == 6 ? smooth_ramp((x - (s.o[0] + (float)s.nx * s.h[0] - s.forc[1])) / s.forc[1]) : 0;
+  return s.forc[2] * gmax(left, s.right_abs * right);
+}
+
+// pressure_geometry.glslinc:39-47 (fractions of the geometry bank)
+__device__ __forceinline__ float geometry_fraction(const Ctx& s, I3 q, int gbank) {
+  q = wrap_cell(s, q);
+  if (q.x < 0 && !solid(s, q)) return coastal_fraction(s, q, s.c[1] + (gbank == candidate(s) ? timestep(s) : 0));
+  if (solid(s, q)) { q.x = iclamp(q.x, 0, s.nx - 1); q.y = iclamp(q.y, 0, s.ny - 1); q.z = iclamp(q.z, 0, s.nz - 1); }
+  if (outside(s, q)) return 0.0f;
+  return getv(s, gbank, index3(q, dims(s)));
+}
+// coupled_motion.glslinc:36-44 (fractions of the current transport sweep)
+__device__ __forceinline__ float transport_f(const Ctx& s, I3 q) {
+  q = wrap_cell(s, q);
+  if (q.x < 0 && !solid(s, q)) return coastal_fraction(s, q, s.c[1]);
+  if (outside(s, q) || solid(s, q)) return 0;
+  int64_t i = index3(q, dims(s)); int sweep = (int)s.c[21];
+  return sweep == 0 ? getv(s, accepted(s), i) : (sweep == 1 ? getv(s, candidate(s), i) : s.motion[i]);
+}
+// source 0: geometry bank; source 1: transport sweep
+__device__ __forceinline__ float hf_fraction(const Ctx& s, I3 q, int source, int gbank) {
+  return source == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+}
+
+// pressure_geometry.glslinc:72-90: weighted Youngs gradient, one-sided next to solids
+__device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source, int gbank) {
+  F3 g = mkf(0, 0, 0);
+  for (int axis = 0; axis < 3; axis++) {
+    int b = (axis + 1) % 3, d = (axis + 2) % 3, weights = 0;
+    float acc = 0;
+    for (int j = -1; j <= 1; j++) for (int k = -1; k <= 1; k++) {
+      I3 middle = q; comp(middle, b) += j; comp(middle, d) += k; if (solid(s, middle)) continue;
+      I3 l = middle, r = middle; comp(l, axis)--; comp(r, axis)++;
+      bool left = !solid(s, l), right = !solid(s, r); if (!left && !right) continue;
+      float difference;
+      if (left && right) difference = hf_fraction(s, l, source, gbank) - hf_fraction(s, r, source, gbank);
+      else if (left) difference = 2 * (hf_fraction(s, l, source, gbank) - hf_fraction(s, middle, source, gbank));
+      else difference = 2 * (hf_fraction(s, middle, source, gbank) - hf_fraction(s, r, source, gbank));
+      int weight = (j == 0 ? 2 : 1) * (k == 0 ? 2 : 1); weights += weight;
+      acc += (float)weight * difference / (32 * s.h[axis]);
+    }
+    if (weights > 0) acc *= 16.0f / (float)weights;
+    comp(g, axis) = acc;
+  }
+  return g;
+}
+__device__ __forceinline__ bool hf_valid(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && !solid(s, q); }
+// height_normal.glslinc:6-34 (numerics flag bit 0)
+__device__ __noinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int source, int gbank) {
+  F3 scaled = mkf(fabsf(youngs.x) * s.h[0], fabsf(youngs.y) * s.h[1], fabsf(youngs.z) * s.h[2]);
+  int axis = scaled.x > scaled.y ? 0 : 1; if (scaled.z > compc(scaled, axis)) axis = 2;
+  if (compc(scaled, axis) < 1e-12f) return youngs;
+  int b = (axis + 1) % 3, d = (axis + 2) % 3; float orientation = gsign(compc(youngs, axis));
+  float heights[5];
+  for (int column = 0; column < 5; column++) {
+    I3 base = q;
+    if (column == 1) comp(base, b)--; if (column == 2) comp(base, b)++;
+    if (column == 3) comp(base, d)--; if (column == 4) comp(base, d)++;
+    float sum = 0, prev = orientation > 0 ? 1 : 0;
+    for (int t = -4; t <= 4; t++) {
+      I3 cell = base; comp(cell, axis) += t; if (!hf_valid(s, cell)) return youngs;
+      float f = hf_fraction(s, cell, source, gbank);
+      if (isnan(f) || isinf(f) || f < 0 || f > 1) return youngs;
+      if (orientation * (f - prev) > 2e-6f) return youngs;
+      if (t == -4 && f != (orientation > 0 ? 1 : 0)) return youngs;
+      if (t == 4 && f != (orientation > 0 ? 0 : 1)) return youngs;
+      sum += f; prev = f;
+    }
+    heights[column] = sum * s.h[axis];
+  }
+  F3 r = mkf(0, 0, 0); comp(r, axis) = orientation;
+  comp(r, b) = -(heights[2] - heights[1]) / (2 * s.h[b]);
+  comp(r, d) = -(heights[4] - heights[3]) / (2 * s.h[d]);
+  return r;
+}
+// pressure_geometry.glslinc:91-98 / coupled_motion.glslinc:50-56
+__device__ __forceinline__ float4 plane_from_gradient(const Ctx& s, I3 q, F3 g, int source, int gbank, float f) {
+  if (s.flags & 1) g = height_normal(s, q, g, source, gbank);
+  if (len3(g) < 1e-9f) g = mkf(0, 1, 0);
+  float L = len3(g);
+  F3 n = mkf(g.x / L * s.h[0], g.y / L * s.h[1], g.z / L * s.h[2]);
+  float l1 = dot3(mkf(fabsf(n.x), fabsf(n.y), fabsf(n.z)), mkf(1, 1, 1));
+  n.x /= l1; n.y /= l1; n.z /= l1;
+  return make_float4(n.x, n.y, n.z, inverse_volume(n, f));
+}
+
+// pressure_geometry.glslinc:99-101
+__device__ __forceinline__ float segment_distance(F3 p, F3 a, F3 b) {
+  F3 ab = mkf(b.x - a.x, b.y - a.y, b.z - a.z), pa = mkf(p.x - a.x, p.y - a.y, p.z - a.z);
+  float t = gclamp(dot3(pa, ab) / gmax(dot3(ab, ab), 1e-30f), 0, 1);
+  return len3(mkf(p.x - (a.x + t * ab.x), p.y - (a.y + t * ab.y), p.z - (a.z + t * ab.z)));
+}
+// pressure_geometry.glslinc:105-120: distance to the PLIC polygon of a cell
+__device__ __noinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq) {
+  F3 h = mkf(s.h[0], s.h[1], s.h[2]);
+  F3 normal = mkf(eq.x / h.x, eq.y / h.y, eq.z / h.z);
+  float norm2 = dot3(normal, normal), step = dot3(normal, point) - eq.w;
+  float signed_d = (dot3(normal, point) - eq.w) / sqrtf(norm2);
+  F3 pr = mkf(point.x - normal.x * step / norm2, point.y - normal.y * step / norm2, point.z - normal.z * step / norm2);
+  if (pr.x >= 0 && pr.y >= 0 && pr.z >= 0 && pr.x <= h.x && pr.y <= h.y && pr.z <= h.z) return fabsf(signed_d);
+  F3 vert[12]; int count = 0;
+  float e3[3] = {eq.x, eq.y, eq.z};
+  for (int axis = 0; axis < 3; axis++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
+    F3 a = mkf(0, 0, 0); comp(a, (axis + 1) % 3) = (float)j; comp(a, (axis + 2) % 3) = (float)k;
+    float den = e3[axis]; if (fabsf(den) < 1e-15f) continue;
+    float t = (eq.w - dot3(mkf(eq.x, eq.y, eq.z), a)) / den;
+    if (t >= 0 && t <= 1) { comp(a, axis) = t; vert[count++] = mkf(a.x * h.x, a.y * h.y, a.z * h.z); }
+  }
+  float best = 1e10f;
+  for (int j = 0; j < count; j++) {
+    best = gmin(best, len3(mkf(point.x - vert[j].x, point.y - vert[j].y, point.z - vert[j].z)));
+    for (int k = 0; k < j; k++) best = gmin(best, segment_distance(point, vert[j], vert[k]));
+  }
+  return best;
+}
+// pressure_geometry.glslinc:124-132
+__device__ __forceinline__ int clip_polygon(F3* poly, int count, float4 eq) {
+  F3 result[12]; int size = 0; F3 n = mkf(eq.x, eq.y, eq.z);
+  for (int j = 0; j < count; j++) {
+    F3 a = poly[j], b = poly[(j + 1) % count];
+    float da = dot3(n, a) - eq.w, db = dot3(n, b) - eq.w;
+    if (da <= 0) result[size++] = a;
+    if ((da < 0 && db > 0) || (da > 0 && db < 0)) {
+      float t = da / (da - db);
+      result[size++] = mkf(a.x * (1 - t) + b.x * t, a.y * (1 - t) + b.y * t, a.z * (1 - t) + b.z * t);
+    }
+  }
+  for (int j = 0; j < size; j++) poly[j] = result[j];
+  return size;
+}
+// pressure_geometry.glslinc:133-156: represented phase transitions on a cell face
+__device__ __noinline__ float transition_distance(const Ctx& s, F3 point, I3 left, int axis, int gbank) {
+  I3 right = left; comp(right, axis)++;
+  I3 l = wrap_cell(s, left), r = wrap_cell(s, right);
+  if (outside(s, l) || outside(s, r) || solid(s, l) || solid(s, r)) return 1e10f;
+  float fl = geometry_fraction(s, l, gbank), fr = geometry_fraction(s, r, gbank);
+  if ((fl <= 0 && fr <= 0) || (fl >= 1 && fr >= 1)) return 1e10f;
+  float4 a = s.plane[index3(l, dims(s))], b = s.plane[index3(r, dims(s))];
+  if (fl <= 0) a = make_float4(0, 0, 0, -1); if (fl >= 1) a = make_float4(0, 0, 0, 1);
+  if (fr <= 0) b = make_float4(0, 0, 0, -1); if (fr >= 1) b = make_float4(0, 0, 0, 1);
+  b.w += axis == 0 ? b.x : (axis == 1 ? b.y : b.z);
+  float best = 1e10f; F3 h = mkf(s.h[0], s.h[1], s.h[2]);
+  for (int side = 0; side < 2; side++) {
+    F3 poly[12]; int j = (axis + 1) % 3, k = (axis + 2) % 3;
+    for (int v = 0; v < 4; v++) { poly[v] = mkf(0, 0, 0); comp(poly[v], axis) = 1; comp(poly[v], j) = (v == 1 || v == 2) ? 1 : 0; comp(poly[v], k) = v >= 2 ? 1 : 0; }
+    float4 first = side == 0 ? a : b;
+    float4 second = side == 0 ? make_float4(-b.x, -b.y, -b.z, -b.w) : make_float4(-a.x, -a.y, -a.z, -a.w);
+    int count = clip_polygon(poly, 4, first); count = clip_polygon(poly, count, second);
+    if (count == 0) continue;
+    F3 projected = point; comp(projected, axis) = compc(h, axis);
+    F3 local = mkf(projected.x / h.x, projected.y / h.y, projected.z / h.z);
+    if (local.x >= 0 && local.y >= 0 && local.z >= 0 && local.x <= 1 && local.y <= 1 && local.z <= 1 &&
+        dot3(mkf(first.x, first.y, first.z), local) <= first.w && dot3(mkf(second.x, second.y, second.z), local) <= second.w)
+      best = gmin(best, fabsf(compc(point, axis) - compc(h, axis)));
+    for (int v = 0; v < count; v++) {
+      F3 p0 = poly[v], p1 = poly[(v + 1) % count];
+      best = gmin(best, segment_distance(point, mkf(p0.x * h.x, p0.y * h.y, p0.z * h.z), mkf(p1.x * h.x, p1.y * h.y, p1.z * h.z)));
+    }
+  }
+  return best;
+}
+// pressure_geometry.glslinc:157-182: exact signed distance at a cell centre
+__device__ __forceinline__ float geometry_distance(const Ctx& s, I3 q, int gbank) {
+  float f = geometry_fraction(s, q, gbank); F3 h = mkf(s.h[0], s.h[1], s.h[2]);
+  if (f > 0 && f < 1) {
+    float4 e = s.plane[index3(q, dims(s))];
+    return (dot3(mkf(e.x, e.y, e.z), mkf(.5f, .5f, .5f)) - e.w) / len3(mkf(e.x / h.x, e.y / h.y, e.z / h.z));
+  }
+  bool needs_seed = false;
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    I3 nb = q; comp(nb, axis) += sign; nb = wrap_cell(s, nb); if (solid(s, nb) || outside(s, nb)) continue;
+    float other = geometry_fraction(s, nb, gbank); bool inside = other >= 1;
+    if (other > 0 && other < 1) { float4 e = s.plane[index3(nb, dims(s))]; inside = dot3(mkf(e.x, e.y, e.z), mkf(.5f, .5f, .5f)) < e.w; }
+    if (inside != (f >= 1)) needs_seed = true;
+  }
+  if (!needs_seed) return (f >= 1 ? -1 : 1) * len3(h) * 4;
+  float best = len3(h) * 4;
+  for (int x = -1; x <= 1; x++) for (int y = -1; y <= 1; y++) for (int z = -1; z <= 1; z++) {
+    I3 nbr = mk(q.x + x, q.y + y, q.z + z), wr = wrap_cell(s, nbr);
+    if (outside(s, wr) || solid(s, wr)) continue;
+    float fraction = geometry_fraction(s, wr, gbank);
+    F3 point = mkf(((float)(q.x - nbr.x) + .5f) * h.x, ((float)(q.y - nbr.y) + .5f) * h.y, ((float)(q.z - nbr.z) + .5f) * h.z);
+    F3 d = mkf(point.x - gclamp(point.x, 0, h.x), point.y - gclamp(point.y, 0, h.y), point.z - gclamp(point.z, 0, h.z));
+    if (len3(d) >= best) continue;
+    if (fraction > 0 && fraction < 1) best = gmin(best, polygon_distance(s, point, s.plane[index3(wr, dims(s))]));
+    for (int axis = 0; axis < 3; axis++) best = gmin(best, transition_distance(s, point, nbr, axis, gbank));
+  }
+  return (f >= 1 ? -1 : 1) * best;
+}

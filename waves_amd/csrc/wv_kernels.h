@@ -1,0 +1,40 @@
+// Host-visible launchers of the gfx950 kernels and the multigrid descriptors.
+#pragma once
+#include "wv_device.h"
+
+#define WV_MAXL 12
+
+// One multigrid level (water/pressure_multigrid.gd:40-69 planning rules).
+struct MGLevel {
+  int n[3];
+  int count;
+  int f[3];        // coarsening factor towards the next level
+  int aggregate;   // transfers to the next level are piecewise constant
+  int width;       // stencil entries of this level's rows
+  int off_start;   // first stencil offset (index into MGDev::offsets / 3)
+  int64_t vec_off; // first float of this level's 8 vector fields
+  int64_t row_off; // first float of this level's k-major rows (levels >= 1)
+};
+struct MGDev {
+  int L, perx;
+  MGLevel lv[WV_MAXL];
+  const int* offsets;        // stencil offsets, 3 ints each
+  const signed char* slots;  // per level: 125 delta slots -> stencil index, -1 if absent
+  float* vec;                // node_count * 8 floats: b, x1, x2, active, l1 D^-1, -, r, d
+  float* rows;               // coarse operator rows, k-major, FP32
+  float* linv;               // last level: dense L^-1 (count^2)
+  float* snapshot;           // 9 N: preconditioner's fine coefficients, kind, face signature
+  float* previous;           // 9 N: previous trial's operator audit copy
+  unsigned* state;           // hierarchy audit header + per-trial records
+  int lag, mode, capacity;
+};
+
+int wv_launch_op(const Ctx& s, int op, hipStream_t st);
+void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st);
+// Multigrid: build = hierarchy audit + Galerkin rebuild (phase 2);
+// cycle = one V-cycle (phase 0 initial / 1 iteration) ending in the fused
+// z write-back, partials and PCG finaliser (op 4 or op 9).
+void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st);
+void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st);
+void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st);
+void wv_launch_spmv_timed(const Ctx& s, hipStream_t st);

@@ -1,0 +1,538 @@
+// Multigrid preconditioner of the pressure solve on gfx950.
+//
+// The hierarchy, transfers, smoother and coarse solve follow
+// water/pressure_multigrid.gd and water/shaders/compute/pressure_multigrid.glslinc,
+// pressure_mg_coarse.glslinc, pressure_hierarchy_state.glsl:
+//   * level 0 applies the lagged snapshot of the fine 7-point coefficients;
+//   * coarse levels hold Galerkin P^T A P rows (k-major, FP32, column implicit in
+//     the stencil offset);
+//   * 3 + 3 sweeps of 4th-kind Chebyshev with l1-Jacobi scaling per level;
+//   * trilinear transfers to levels 1 and 2, aggregation below;
+//   * dense Cholesky (stored as L^-1) on the last level (<= 64 nodes).
+// The reference assembles P^T A P by scattering float CAS atomics
+// (pressure_multigrid.glslinc:128-175), whose order is nondeterministic.  Here
+// every coarse row gathers its own entries (one thread per coarse row, the
+// accumulator in LDS), then the lower triangle is copied from the upper one so
+// the coarse operator is exactly symmetric, as the reference's canonical upper
+// triangle storage is.  The build is deterministic.
+#include "wv_kernels.h"
+
+#define GID() ((int64_t)blockIdx.x * blockDim.x + threadIdx.x)
+
+__device__ __forceinline__ int pmod(int x, int n) { return x >= 0 ? x % n : (n - 1) - ((-1 - x) % n); }
+__device__ __forceinline__ int64_t at(const MGDev& m, int l, int field, int64_t i) { return m.lv[l].vec_off + (int64_t)field * m.lv[l].count + i; }
+__device__ __forceinline__ I3 mdims(const MGDev& m, int l) { return mk(m.lv[l].n[0], m.lv[l].n[1], m.lv[l].n[2]); }
+// pressure_multigrid.glslinc:36-47
+__device__ __forceinline__ int mwrap(const MGDev& m, int l, I3 q) {
+  I3 n = mdims(m, l); q.z = pmod(q.z, n.z);
+  if (m.perx) q.x = pmod(q.x, n.x);
+  if (q.x < 0 || q.x >= n.x || q.y < 0 || q.y >= n.y) return -1;
+  return (int)index3(q, n);
+}
+__device__ __forceinline__ I3 mdelta(const MGDev& m, int l, int i, int j) {
+  I3 n = mdims(m, l), a = coord(i, n), b = coord(j, n);
+  I3 d = mk(b.x - a.x, b.y - a.y, b.z - a.z);
+  if (m.perx) d.x = pmod(d.x + n.x / 2, n.x) - n.x / 2;
+  d.z = pmod(d.z + n.z / 2, n.z) - n.z / 2;
+  return d;
+}
+__device__ __forceinline__ int slot125(I3 d) { return d.x + 2 + 5 * (d.y + 2 + 5 * (d.z + 2)); }
+__device__ __forceinline__ bool in_box2(I3 d) { return abs(d.x) <= 2 && abs(d.y) <= 2 && abs(d.z) <= 2; }
+__device__ __forceinline__ I3 moffset(const MGDev& m, int l, int k) { const int* o = m.offsets + 3 * (m.lv[l].off_start + k); return mk(o[0], o[1], o[2]); }
+__device__ __forceinline__ bool mactive(const MGDev& m, int l, int i) { return i >= 0 && m.vec[at(m, l, 3, i)] > .5f; }
+__device__ __forceinline__ float snap(const MGDev& m, int field, int64_t i) { return m.snapshot[(int64_t)(field - 6) * m.lv[0].count + i]; }
+
+// pressure_multigrid.glslinc:90-105
+__device__ __forceinline__ float mapply(const MGDev& m, int l, int field, int i) {
+  if (!mactive(m, l, i)) return m.vec[at(m, l, field, i)];
+  float sum = 0; I3 q = coord(i, mdims(m, l));
+  if (l == 0) {
+    float here = m.vec[at(m, 0, field, i)];
+#pragma unroll
+    for (int f6 = 0; f6 < 6; f6++) {
+      int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+      int j = mwrap(m, 0, nb);
+      float other = mactive(m, 0, j) ? m.vec[at(m, 0, field, j)] : 0;
+      sum += snap(m, 6 + f6, i) * (here - other);
+    }
+    return sum;
+  }
+  int width = m.lv[l].width, N = m.lv[l].count; const float* row = m.rows + m.lv[l].row_off + i;
+  for (int k = 0; k < width; k++) {
+    I3 off = moffset(m, l, k); int j = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
+    sum += row[(int64_t)k * N] * m.vec[at(m, l, field, j >= 0 ? j : 0)];
+  }
+  return sum;
+}
+// pressure_multigrid.glslinc:108-120: transfers of fine node i (level l) to level l+1
+__device__ __forceinline__ int mtransfers(const MGDev& m, int l, int i, int* ids, float* w) {
+  const MGLevel& L = m.lv[l]; I3 n = mdims(m, l + 1), q = coord(i, mdims(m, l));
+  if (L.aggregate) {
+    int j = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), n);
+    if (!mactive(m, l + 1, j)) return 0; ids[0] = j; w[0] = 1; return 1;
+  }
+  int base[3]; float t[3]; int qq[3] = {q.x, q.y, q.z};
+  for (int k = 0; k < 3; k++) { float pos = ((float)qq[k] + .5f) / (float)L.f[k] - .5f; base[k] = (int)floorf(pos); t[k] = fract(pos); }
+  int size = 0;
+  for (int z = 0; z < L.f[2]; z++) for (int y = 0; y < L.f[1]; y++) for (int x = 0; x < L.f[0]; x++) {
+    int bit[3] = {x, y, z}; I3 nb = mk(base[0] + x, base[1] + y, base[2] + z); float ww = 1;
+    for (int axis = 0; axis < 3; axis++) if (L.f[axis] > 1) ww *= bit[axis] == 1 ? t[axis] : 1 - t[axis];
+    if (nb.y >= n.y) continue; if (nb.y < 0) nb.y = 0;
+    nb.x = m.perx ? pmod(nb.x, n.x) : iclamp(nb.x, 0, n.x - 1); nb.z = pmod(nb.z, n.z);
+    int j = (int)index3(nb, n); if (!mactive(m, l + 1, j) || ww == 0) continue;
+    ids[size] = j; w[size] = ww; size++;
+  }
+  return size;
+}
+__device__ __forceinline__ bool mg_enabled(const Ctx& s, int phase) {
+  const float* c = s.c;
+  if (c[7] > .5f) return false;
+  if (phase == 0) return (c[5] > .5f || c[57] == 2) && c[6] < .5f;
+  return c[4] > .5f && c[6] < .5f;
+}
+
+// ------------------------------------------------------------ V-cycle -------
+// pressure_multigrid.glslinc:191 (op 5): b = r, x1 = x2 = 0 on level 0
+__global__ void k_mg_load(Ctx s, MGDev m, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[0].count) return;
+  m.vec[at(m, 0, 0, i)] = s.work[i]; m.vec[at(m, 0, 1, i)] = 0; m.vec[at(m, 0, 2, i)] = 0;
+}
+// pressure_multigrid.glslinc:192-196: 4th-kind Chebyshev sweep k, l1-Jacobi scaled
+__global__ void k_mg_cheb(Ctx s, MGDev m, int l, int source, int k, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
+  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
+  m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + d;
+}
+// terminal smoother when coarsening stops above 64 nodes
+// (water/pressure_multigrid.gd:144-149): op 14 then op 6 sweeps
+__global__ void k_mg_jacobi(Ctx s, MGDev m, int l, int source, int first, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  if (first) { m.vec[at(m, l, 2, i)] = m.vec[at(m, l, 4, i)] * m.vec[at(m, l, 0, i)]; return; }
+  m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + m.vec[at(m, l, 4, i)] * (m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i));
+}
+// op 7
+__global__ void k_mg_residual(Ctx s, MGDev m, int l, int field, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  m.vec[at(m, l, 6, i)] = m.vec[at(m, l, 0, i)] - mapply(m, l, field, (int)i);
+}
+// op 8: exact P^T residual gather into level l from level l-1
+__global__ void k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  int parent = l - 1; const MGLevel& P = m.lv[parent];
+  I3 nf = mdims(m, parent), nl = mdims(m, l), qc = coord(i, nl);
+  float sum = 0;
+  if (P.aggregate) {
+    if (mactive(m, l, (int)i))
+      for (int z = 0; z < P.f[2]; z++) for (int y = 0; y < P.f[1]; y++) for (int x = 0; x < P.f[0]; x++) {
+        int j = mwrap(m, parent, mk(qc.x * P.f[0] + x, qc.y * P.f[1] + y, qc.z * P.f[2] + z));
+        if (mactive(m, parent, j)) sum += m.vec[at(m, parent, 6, j)];
+      }
+  } else {
+    int lo[3] = {qc.x * P.f[0], qc.y * P.f[1], qc.z * P.f[2]}, ext[3] = {1, 1, 1}, nfa[3] = {nf.x, nf.y, nf.z}, nla[3] = {nl.x, nl.y, nl.z}, qca[3] = {qc.x, qc.y, qc.z};
+    for (int axis = 0; axis < 3; axis++) if (P.f[axis] > 1) {
+      lo[axis]--; ext[axis] = 4;
+      if ((axis == 2 || (axis == 0 && m.perx)) && nfa[axis] <= 4) { lo[axis] = 0; ext[axis] = nfa[axis]; }
+    }
+    if (mactive(m, l, (int)i))
+      for (int z = 0; z < ext[2]; z++) for (int y = 0; y < ext[1]; y++) for (int x = 0; x < ext[0]; x++) {
+        int j = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)); if (!mactive(m, parent, j)) continue;
+        I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z}; float weight = 1;
+        for (int axis = 0; axis < 3; axis++) {
+          float pos = ((float)qa[axis] + .5f) / (float)P.f[axis] - .5f; int base = (int)floorf(pos); float t = fract(pos);
+          float w = 0;
+          for (int bit = 0; bit < P.f[axis]; bit++) {
+            int nb = base + bit;
+            if (axis == 2 || (axis == 0 && m.perx)) nb = pmod(nb, nla[axis]);
+            else if (axis == 0) nb = iclamp(nb, 0, nla[axis] - 1); else nb = nb > 0 ? nb : 0;
+            if (nb == qca[axis]) w += P.f[axis] == 1 ? 1 : (bit == 1 ? t : 1 - t);
+          }
+          weight *= w;
+        }
+        sum += weight * m.vec[at(m, parent, 6, j)];
+      }
+  }
+  m.vec[at(m, l, 0, i)] = sum; m.vec[at(m, l, 1, i)] = 0; m.vec[at(m, l, 2, i)] = 0;
+}
+// op 9: x_l += P x_{l+1}
+__global__ void k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  float add = 0;
+  if (mactive(m, l, (int)i)) { int ids[8]; float ws[8]; int size = mtransfers(m, l, (int)i, ids, ws); for (int k = 0; k < size; k++) add += ws[k] * m.vec[at(m, l + 1, cf, ids[k])]; }
+  m.vec[at(m, l, ff, i)] += add;
+}
+// pressure_mg_coarse.glslinc:50-53 (op 12): z = L^-T (L^-1 b), one group
+__global__ void k_mg_coarse_solve(Ctx s, MGDev m, int l, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  __shared__ float rhs[64];
+  int lid = threadIdx.x, N = m.lv[l].count; const float* Li = m.linv;
+  if (lid < N) { float d = 0; for (int j = 0; j <= lid; j++) d += Li[lid * N + j] * m.vec[at(m, l, 0, j)]; rhs[lid] = d; }
+  __syncthreads();
+  if (lid < N) { float d = 0; for (int j = lid; j < N; j++) d += Li[j * N + lid] * rhs[j]; m.vec[at(m, l, 1, lid)] = d; }
+}
+// pressure_multigrid.glslinc:230-234 (op 10) fused with the PCG finaliser:
+// phase 0 -> pressure_iterations.glslinc:14-19 (op 4), phase 1 -> :31 (op 9)
+__global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, int phase, unsigned* ticket) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); int64_t N = s.N; float4 value = make_float4(0, 0, 0, 0);
+  if (i < N) {
+    float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i];
+    s.work[N + i] = z; if (phase == 0) s.work[2 * N + i] = z;
+    value = make_float4(r * z, r * r, b * b, fabsf(r));
+  }
+  value = group_reduce<RED_PRESSURE>(value);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = value;
+  if (!last_block(ticket)) return;
+  float4 v = global_reduce<true>(s.partial, gridDim.x);
+  if (threadIdx.x != 0) return;
+  float* c = s.c;
+  if (phase == 0) {
+    c[9] = v.x; c[10] = v.z; c[13] = sqrtf(v.y / gmax(v.z, 1e-30f));
+    if (residual_converged(s, v, v.z)) c[6] = 1;
+    if (c[58] == 0) c[59] = v.z;
+    c[57] = 0;
+  } else {
+    c[12] = v.x / gmax(c[9], 1e-30f); c[9] = v.x; c[8]++; c[18]++;
+    c[13] = sqrtf(v.y / gmax(c[10], 1e-30f));
+    if (isnan(c[13]) || isinf(c[13])) c[7] = 2;
+    else if (residual_converged(s, v, c[10])) c[6] = 1;
+    else if (c[8] > 4096) c[7] = 3;
+  }
+}
+// op 3 when the V-cycle is predicated off (a failure code is set): the
+// reference's op 4 then reduces op 3's own partials.
+__global__ void __launch_bounds__(WV_RED) k_pcg_init_partials(Ctx s, unsigned* ticket) {
+  if (!pressure_setup(s)) return;
+  if (mg_enabled(s, 0)) return;
+  int64_t i = GID(); int64_t N = s.N; float4 v = make_float4(0, 0, 0, 0);
+  if (i < N) { float r = s.work[i], z = s.work[N + i], bv = s.work[5 * N + i]; v = make_float4(r * z, r * r, bv * bv, fabsf(r)); }
+  v = group_reduce<RED_PRESSURE>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<true>(s.partial, gridDim.x);
+  if (threadIdx.x != 0) return;
+  float* c = s.c;
+  c[9] = g.x; c[10] = g.z; c[13] = sqrtf(g.y / gmax(g.z, 1e-30f));
+  if (residual_converged(s, g, g.z)) c[6] = 1;
+  if (c[58] == 0) c[59] = g.z;
+  c[57] = 0;
+}
+
+// ---------------------------------------------------- hierarchy audit -------
+// pressure_hierarchy_state.glsl:60-68
+__device__ __forceinline__ unsigned signature(const Ctx& s, int64_t i) {
+  if (s.geo[i].y < .5f) return 0;
+  I3 n = dims(s), q = coord(i, n); unsigned result = 0;
+  for (int f6 = 0; f6 < 6; f6++) {
+    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+    nb.z = pmod(nb.z, n.z); if (s.perx > .5f) nb.x = pmod(nb.x, n.x);
+    bool inside = nb.x >= 0 && nb.x < n.x && nb.y >= 0 && nb.y < n.y;
+    bool w = inside && s.geo[index3(nb, n)].y > .5f;
+    unsigned kind = s.work[(int64_t)(6 + f6) * s.N + i] > 0 ? (w ? 1u : 2u) : 0u;
+    result |= kind << (2 * f6);
+  }
+  return result;
+}
+__device__ __forceinline__ bool audit_on(const Ctx& s) { return !(s.c[5] < .5f || s.c[7] > .5f); }
+__global__ void k_audit0(Ctx s, MGDev m) {
+  if (!audit_on(s)) return;
+  unsigned* st = m.state;
+  for (int k = 1; k <= 7; k++) st[k] = 0; st[14] = st[0]; st[15] = 0; st[16] = 0; for (int k = 20; k < 32; k++) st[k] = 0;
+}
+__global__ void k_audit1(Ctx s, MGDev m) {
+  if (!audit_on(s)) return;
+  __shared__ unsigned cnt[9];
+  if (threadIdx.x < 9) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t i = GID(), n = s.N;
+  if (i < n) {
+    const float* cached = m.snapshot; const float* prev = m.previous;
+    float kind = s.geo[i].y;
+    unsigned a_x = kind != cached[7 * n + i], a_w = kind != prev[7 * n + i], d_y = kind > .5f;
+    unsigned sig = signature(s, i), old = (unsigned)cached[8 * n + i], pv = (unsigned)prev[8 * n + i];
+    unsigned a_y = 0, b_x = 0, a_z = 0, b_y = 0, d_x = 0; float ratio = 0;
+    for (int face = 0; face < 6; face++) { a_y += ((sig >> (2 * face)) & 3) != ((old >> (2 * face)) & 3); b_x += ((sig >> (2 * face)) & 3) != ((pv >> (2 * face)) & 3); }
+    for (int k = 0; k < 7; k++) {
+      float value = s.work[(6 + k) * n + i], was = cached[k * n + i];
+      a_z += __float_as_uint(value) != __float_as_uint(was); b_y += __float_as_uint(value) != __float_as_uint(prev[k * n + i]);
+      if (was > 0) ratio = gmax(ratio, fabsf(value - was) / was);
+      if (isnan(value) || isinf(value) || value < 0) d_x = 1;
+    }
+    if (a_x) atomicAdd(&cnt[0], a_x); if (a_y) atomicAdd(&cnt[1], a_y); if (a_z) atomicAdd(&cnt[2], a_z); if (a_w) atomicAdd(&cnt[3], a_w);
+    if (b_x) atomicAdd(&cnt[4], b_x); if (b_y) atomicAdd(&cnt[5], b_y); atomicMax(&cnt[6], __float_as_uint(ratio));
+    if (d_x) atomicAdd(&cnt[7], d_x); if (d_y) atomicAdd(&cnt[8], d_y);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned* st = m.state;
+    for (int k = 0; k < 4; k++) if (cnt[k]) atomicAdd(&st[1 + k], cnt[k]);
+    if (cnt[4]) atomicAdd(&st[5], cnt[4]); if (cnt[5]) atomicAdd(&st[6], cnt[5]); atomicMax(&st[7], cnt[6]);
+    if (cnt[7]) atomicAdd(&st[15], cnt[7]); if (cnt[8]) atomicAdd(&st[16], cnt[8]);
+  }
+}
+__global__ void k_audit2(Ctx s, MGDev m) {
+  if (!audit_on(s)) return;
+  unsigned* st = m.state;
+  if (st[15] > 0) { s.c[7] = 34; return; }
+  int mode = m.mode;
+  bool incompatible = st[0] == 0 || st[1] > 0 || st[2] > 0;
+  bool structural = mode == 0 || incompatible;
+  bool numerical = mode == 0 || incompatible || (st[3] > 0 && (mode == 1 || st[10] + 1 >= (unsigned)m.lag));
+  st[8] = numerical; st[9] = structural; st[10] = numerical || st[3] == 0 ? 0 : st[10] + 1; st[11] += st[8]; st[12] += st[9]; st[13]++; st[0] = 1;
+  if (st[13] <= (unsigned)m.capacity) {
+    int64_t row = 64 + 32 * ((int64_t)st[13] - 1); st[row] = st[13]; st[row + 1] = __float_as_uint(s.c[1]);
+    for (int k = 1; k <= 7; k++) st[row + 1 + k] = st[k];
+    st[row + 9] = st[8]; st[row + 10] = st[9]; st[row + 11] = st[10]; st[row + 12] = st[16]; st[row + 13] = st[11];
+  }
+}
+__global__ void k_audit3(Ctx s, MGDev m) {
+  if (!audit_on(s)) return;
+  int64_t i = GID(), n = s.N; if (i >= n) return;
+  bool numeric = m.state[8] != 0;
+  for (int k = 0; k < 7; k++) { float value = s.work[(6 + k) * n + i]; m.previous[k * n + i] = value; if (numeric) m.snapshot[k * n + i] = value; }
+  float kind = s.geo[i].y, sig = (float)signature(s, i);
+  m.previous[7 * n + i] = kind; m.previous[8 * n + i] = sig;
+  if (numeric) { m.snapshot[7 * n + i] = kind; m.snapshot[8 * n + i] = sig; }
+}
+__global__ void k_audit4(Ctx s, MGDev m) {
+  if (!audit_on(s)) return;
+  unsigned* st = m.state;
+  if (st[13] <= (unsigned)m.capacity) { int64_t row = 64 + 32 * ((int64_t)st[13] - 1); for (int l = 0; l < 12; l++) st[row + 16 + l] = st[20 + l]; }
+}
+
+// ------------------------------------------------------------ MG build ------
+__device__ __forceinline__ bool build_on(const Ctx& s, const MGDev& m, bool structural) {
+  if (s.c[7] > .5f || s.c[5] < .5f) return false;
+  return structural ? m.state[9] != 0 : m.state[8] != 0;
+}
+// pressure_multigrid.glslinc:139-150 (op 1)
+__global__ void k_mg_classify(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, true)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  float value = 0;
+  if (l == 0) value = s.geo[i].y > .5f ? 1 : 0;
+  else {
+    const MGLevel& P = m.lv[l - 1]; I3 base = coord(i, mdims(m, l));
+    base.x *= P.f[0]; base.y *= P.f[1]; base.z *= P.f[2];
+    for (int z = 0; z < P.f[2]; z++) for (int y = 0; y < P.f[1]; y++) for (int x = 0; x < P.f[0]; x++)
+      if (mactive(m, l - 1, mwrap(m, l - 1, mk(base.x + x, base.y + y, base.z + z)))) value = 1;
+  }
+  if (m.vec[at(m, l, 3, i)] != value) atomicAdd(&m.state[20 + l], 1u);
+  m.vec[at(m, l, 3, i)] = value;
+}
+// pressure_multigrid.glslinc:183-189 (op 4), level 0
+__global__ void k_mg_l1_fine(Ctx s, MGDev m) {
+  if (!build_on(s, m, false)) return;
+  int64_t i = GID(); if (i >= m.lv[0].count) return;
+  I3 q = coord(i, mdims(m, 0)); float sum = snap(m, 12, i);
+  for (int f6 = 0; f6 < 6; f6++) {
+    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+    if (mactive(m, 0, mwrap(m, 0, nb))) sum += snap(m, 6 + f6, i);
+  }
+  if (!(sum > 0) || isnan(sum) || isinf(sum)) { s.c[7] = 32; return; }
+  m.vec[at(m, 0, 4, i)] = 1 / sum;
+}
+// op 4 at a coarse level: l1 row sums of the (symmetric) Galerkin rows
+__global__ void k_mg_l1_coarse(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  int64_t i = GID(); int N = m.lv[l].count; if (i >= N) return;
+  const float* row = m.rows + m.lv[l].row_off + i; float sum = 0;
+  for (int k = 0; k < m.lv[l].width; k++) sum += fabsf(row[(int64_t)k * N]);
+  if (!(sum > 0) || isnan(sum) || isinf(sum)) { s.c[7] = 32; return; }
+  m.vec[at(m, l, 4, i)] = 1 / sum;
+}
+
+#define GAL_THREADS 64
+// Galerkin rows of level l+1 from level l: row I gathers
+//   C[I, J] = sum_i sum_j P[i, I] A[i, j] P[j, J]
+// over the fine nodes i whose transfers reach I.  Accumulators live in LDS.
+__global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  extern __shared__ float acc[];  // width(l+1) * GAL_THREADS
+  int lc = l + 1; const MGLevel& C = m.lv[lc]; const MGLevel& F = m.lv[l];
+  int64_t I = GID(); int tid = threadIdx.x; int W = C.width;
+  for (int k = 0; k < W; k++) acc[k * GAL_THREADS + tid] = 0;
+  const signed char* slots = m.slots + 125 * lc;
+  bool live = I < C.count && mactive(m, lc, (int)I);
+  if (live) {
+    I3 qc = coord(I, mdims(m, lc)); I3 nf = mdims(m, l);
+    int lo[3], hi[3]; int qca[3] = {qc.x, qc.y, qc.z}; int nfa[3] = {nf.x, nf.y, nf.z};
+    for (int a = 0; a < 3; a++) {
+      if (F.aggregate) { lo[a] = qca[a] * F.f[a]; hi[a] = lo[a] + F.f[a] - 1; if (hi[a] > nfa[a] - 1) hi[a] = nfa[a] - 1; }
+      else if (F.f[a] == 1) { lo[a] = qca[a]; hi[a] = qca[a]; }
+      else { lo[a] = 2 * qca[a] - 1; hi[a] = 2 * qca[a] + 2;
+        bool periodic = a == 2 || (a == 0 && m.perx);
+        if (!periodic) { if (lo[a] < 0) lo[a] = 0; if (hi[a] > nfa[a] - 1) hi[a] = nfa[a] - 1; }
+      }
+    }
+    for (int z = lo[2]; z <= hi[2]; z++) for (int y = lo[1]; y <= hi[1]; y++) for (int x = lo[0]; x <= hi[0]; x++) {
+      int i = mwrap(m, l, mk(x, y, z)); if (i < 0 || !mactive(m, l, i)) continue;
+      int ids[8]; float ws[8]; int ni = mtransfers(m, l, i, ids, ws);
+      float wI = 0; for (int a = 0; a < ni; a++) if (ids[a] == (int)I) wI += ws[a];
+      if (wI == 0) continue;
+      I3 q = coord(i, nf);
+      int width = l == 0 ? 7 : F.width;
+      for (int k = 0; k < width; k++) {
+        int j; float aij;
+        if (l == 0) {
+          if (k == 0) { j = i; aij = snap(m, 12, i); }
+          else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1; j = mwrap(m, 0, nb); if (j < 0) continue; aij = -snap(m, 6 + f6, i); }
+        } else {
+          I3 off = moffset(m, l, k); j = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z)); if (j < 0) continue;
+          aij = m.rows[F.row_off + (int64_t)k * F.count + i];
+        }
+        if (aij == 0 || !mactive(m, l, j)) continue;
+        int jd[8]; float jw[8]; int nj = mtransfers(m, l, j, jd, jw);
+        for (int b = 0; b < nj; b++) {
+          I3 d = mdelta(m, lc, (int)I, jd[b]);
+          int sl = in_box2(d) ? slots[slot125(d)] : -1;
+          if (sl < 0) { s.c[7] = 31; continue; }
+          acc[sl * GAL_THREADS + tid] += wI * aij * jw[b];
+        }
+      }
+    }
+  }
+  if (I >= C.count) return;
+  float* row = m.rows + C.row_off + I;
+  for (int k = 0; k < W; k++) {
+    I3 off = moffset(m, lc, k);
+    float v = live ? acc[k * GAL_THREADS + tid] : ((off.x == 0 && off.y == 0 && off.z == 0) ? 1.0f : 0.0f);
+    row[(int64_t)k * C.count] = v;
+  }
+}
+// Keep the canonical (upper) entry of every coupling and mirror it, so the
+// coarse operator is exactly symmetric; non-canonical duplicate offsets stay 0.
+__global__ void k_mg_symmetrize(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  int64_t I = GID(); const MGLevel& C = m.lv[l]; if (I >= C.count) return;
+  I3 q = coord(I, mdims(m, l)); const signed char* slots = m.slots + 125 * l;
+  float* base = m.rows + C.row_off;
+  for (int k = 0; k < C.width; k++) {
+    I3 off = moffset(m, l, k); int J = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
+    if (J < 0) { base[(int64_t)k * C.count + I] = 0; continue; }
+    I3 d = mdelta(m, l, (int)I, J);
+    if (d.x != off.x || d.y != off.y || d.z != off.z) { base[(int64_t)k * C.count + I] = 0; continue; }
+    if (J < I) {
+      I3 e = mdelta(m, l, J, (int)I); int sl = slots[slot125(e)];
+      base[(int64_t)k * C.count + I] = base[(int64_t)sl * C.count + J];
+    }
+  }
+}
+// pressure_mg_coarse.glslinc:35-49 (op 11): dense Cholesky in LDS, store L^-1
+__global__ void k_mg_cholesky(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  __shared__ float dense[64 * 64], inv[64 * 64];
+  int lid = threadIdx.x, N = m.lv[l].count; const signed char* slots = m.slots + 125 * l;
+  for (int k = lid; k < N * N; k += blockDim.x) {
+    int i = k / N, j = k % N; float v;
+    if (!mactive(m, l, i) || !mactive(m, l, j)) v = i == j ? 1 : 0;
+    else {
+      int a = i < j ? i : j, b = i < j ? j : i; I3 d = mdelta(m, l, a, b);
+      int sl = in_box2(d) ? slots[slot125(d)] : -1;
+      v = sl < 0 ? 0 : m.rows[m.lv[l].row_off + (int64_t)sl * N + a];
+    }
+    dense[k] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < N; k++) {
+    if (lid == 0) {
+      float d = dense[k * N + k]; for (int j = 0; j < k; j++) d -= dense[k * N + j] * dense[k * N + j];
+      if (!(d > 0) || isnan(d) || isinf(d)) { s.c[7] = 33; d = 1; }
+      dense[k * N + k] = sqrtf(d);
+    }
+    __syncthreads();
+    if (lid > k && lid < N) { float d = dense[lid * N + k]; for (int j = 0; j < k; j++) d -= dense[lid * N + j] * dense[k * N + j]; dense[lid * N + k] = d / dense[k * N + k]; }
+    __syncthreads();
+  }
+  for (int row = 0; row < N; row++) {
+    if (lid < N) { float d = lid == row ? 1 : 0; for (int j = 0; j < row; j++) d -= dense[row * N + j] * inv[j * N + lid]; inv[row * N + lid] = d / dense[row * N + row]; }
+    __syncthreads();
+  }
+  for (int k = lid; k < N * N; k += blockDim.x) m.linv[k] = inv[k];
+}
+
+static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// water/pressure_multigrid.gd:121-128 + water/pressure_hierarchy_state.gd:36-41
+void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
+  hipLaunchKernelGGL(k_audit0, dim3(1), dim3(1), 0, st, s, m);
+  hipLaunchKernelGGL(k_audit1, dim3(nblocks(s.N, 256)), dim3(256), 0, st, s, m);
+  hipLaunchKernelGGL(k_audit2, dim3(1), dim3(1), 0, st, s, m);
+  hipLaunchKernelGGL(k_audit3, dim3(nblocks(s.N, 256)), dim3(256), 0, st, s, m);
+  for (int l = 0; l < m.L; l++) hipLaunchKernelGGL(k_mg_classify, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l);
+  hipLaunchKernelGGL(k_mg_l1_fine, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m);
+  for (int l = 0; l + 1 < m.L; l++) {
+    size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
+    hipLaunchKernelGGL(k_mg_galerkin, dim3(nblocks(m.lv[l + 1].count, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
+    hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
+    hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
+  }
+  if (m.lv[m.L - 1].count <= 64) hipLaunchKernelGGL(k_mg_cholesky, dim3(1), dim3(64), 0, st, s, m, m.L - 1);
+  hipLaunchKernelGGL(k_audit4, dim3(1), dim3(1), 0, st, s, m);
+}
+
+struct CycleBuilder {
+  const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
+  int smooth(int l, int field) {
+    for (int sweep = 0; sweep < smoothing; sweep++) {
+      hipLaunchKernelGGL(k_mg_cheb, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      field = 3 - field;
+    }
+    return field;
+  }
+  // water/pressure_multigrid.gd:141-154
+  int cycle(int l) {
+    if (l == m.L - 1) {
+      if (m.lv[l].count <= 64) { hipLaunchKernelGGL(k_mg_coarse_solve, dim3(1), dim3(64), 0, st, s, m, l, phase); return 1; }
+      int tf = 1;
+      for (int sweep = 0; sweep < 32; sweep++) {
+        hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, sweep == 0 ? 1 : 0, phase);
+        tf = 3 - tf;
+      }
+      return tf;
+    }
+    int field = smooth(l, 1);
+    hipLaunchKernelGGL(k_mg_residual, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1, phase);
+    int cf = cycle(l + 1);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
+    return smooth(l, field);
+  }
+};
+
+void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st) {
+  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
+  CycleBuilder b{s, m, phase, smoothing, st};
+  int final_field = b.cycle(0);
+  hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, s.counters + 16 + phase);
+}
+
+// op 3's partials + op 4 when the V-cycle is predicated off
+void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_init_partials, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, s.counters + 20);
+}
+
+// Hot kernels launched on their own for live timing (the kernels' outputs are
+// V-cycle / PCG scratch that every use recomputes first); phase 3 = ungated.
+__global__ void k_mg_cheb_timed(Ctx s, MGDev m) {
+  int64_t i = GID(); if (i >= m.lv[0].count) return;
+  int l = 0, source = 1, k = 2;
+  float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
+  float d = (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)] + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
+  m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 2, i)] = m.vec[at(m, l, 1, i)] + d;
+}
+__global__ void k_mg_residual_timed(Ctx s, MGDev m) {
+  int64_t i = GID(); if (i >= m.lv[0].count) return;
+  m.vec[at(m, 0, 6, i)] = m.vec[at(m, 0, 0, i)] - mapply(m, 0, 1, (int)i);
+}
+void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
+  if (kernel == 0) hipLaunchKernelGGL(k_mg_cheb_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
+  else if (kernel == 2) hipLaunchKernelGGL(k_mg_residual_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
+  else wv_launch_spmv_timed(c, st);
+}

@@ -1,0 +1,187 @@
+// Face-velocity sampling, MacCormack traces, CFL rates and air extension on the
+// device.  Restates water/shaders/compute/coupled_motion.glslinc:15-230.
+#pragma once
+#include "wv_geom.h"
+
+__device__ __forceinline__ F3 periodic_delta(const Ctx& s, I3 a, I3 b) {
+  F3 d = mkf((float)(a.x - b.x), (float)(a.y - b.y), (float)(a.z - b.z));
+  d.z -= roundf(d.z / (float)s.nz) * (float)s.nz;
+  if (s.perx > .5f) d.x -= roundf(d.x / (float)s.nx) * (float)s.nx;
+  return mkf(d.x * s.h[0], d.y * s.h[1], d.z * s.h[2]);
+}
+__device__ __forceinline__ bool wall_face(const Ctx& s, I3 q, int axis) { I3 l = q; comp(l, axis)--; return solid(s, q) || solid(s, l); }
+__device__ __forceinline__ bool face_in_bounds(const Ctx& s, I3 q, int axis) {
+  I3 n = dims(s); comp(n, axis)++;
+  return q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < n.x && q.y < n.y && q.z < n.z;
+}
+__device__ __forceinline__ I3 wrap_face(const Ctx& s, I3 q) { q.z = pidx(q.z, s.nz); if (s.perx > .5f) q.x = pidx(q.x, s.nx); return q; }
+// coupled_motion.glslinc:28-35: accepted face velocity with the ocean ghost
+__device__ __forceinline__ float old_face(const Ctx& s, I3 q, int axis) {
+  q = wrap_face(s, q);
+  if (q.x < 0 && !wall_face(s, q, axis)) return coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1], axis);
+  if (!face_in_bounds(s, q, axis) || wall_face(s, q, axis)) return 0;
+  return getv(s, accepted(s), face_index(s, q, axis));
+}
+__device__ __forceinline__ float face_courant(const Ctx& s, I3 q, int axis) { return old_face(s, q, axis) * timestep(s) / s.h[axis]; }
+
+// coupled_motion.glslinc:66-85: nearest wall within three layers
+__device__ __noinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int component, float& parity) {
+  float best = 1e20f; I3 selected = q; parity = 1;
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    for (int layer = 1; layer <= 3; layer++) {
+      I3 edge = q; comp(edge, axis) += sign * layer; edge = wrap_face(s, edge);
+      if (!face_in_bounds(s, edge, component)) break;
+      I3 left = edge; comp(left, component)--;
+      bool sl = solid(s, left), sr = solid(s, edge);
+      bool boundary = axis == component ? (sl != sr) : (!sl && !sr);
+      if (!boundary) continue;
+      I3 mirror = q; comp(mirror, axis) += sign * (2 * layer - (axis == component ? 0 : 1)); mirror = wrap_face(s, mirror);
+      float distance = ((float)layer - (axis == component ? 0 : .5f)) * s.h[axis];
+      if (face_in_bounds(s, mirror, component) && !wall_face(s, mirror, component) && distance < best) {
+        selected = mirror; best = distance; parity = axis == component ? -1 : 1;
+      }
+      break;
+    }
+  }
+  q = selected; return best < 1e19f;
+}
+// coupled_motion.glslinc:86-111
+__device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bool forward, bool& valid) {
+  float parity = 1;
+  if (q.y < 0) { q.y = -q.y - (axis == 1 ? 0 : 1); if (axis == 1) parity = -parity; }
+  if (s.perx < .5f) {
+    if (q.x >= s.nx + (axis == 0 ? 1 : 0)) { q.x = 2 * s.nx - q.x - (axis == 0 ? 0 : 1); if (axis == 0) parity = -parity; }
+  }
+  q = wrap_face(s, q);
+  if (q.x < 0 && q.y >= 0 && q.y < s.ny && !wall_face(s, q, axis)) {
+    valid = true; return parity * coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1], axis);
+  }
+  if (!face_in_bounds(s, q, axis)) { valid = false; return 0; }
+  I3 l = q; comp(l, axis)--; bool left = solid(s, l), right = solid(s, q);
+  if (left != right) { valid = true; return 0; }
+  if (left && right) {
+    float reflection;
+    if (!reflect_voxel_donor(s, q, axis, reflection)) { valid = false; return 0; }
+    parity *= reflection;
+  }
+  int acc = accepted(s);
+  int64_t f = face_local(s, q, axis); valid = s.motion[valid_base(s, acc) + f] > .5f;
+  return parity * (forward ? s.motion[forward_base(s) + f] : getv(s, acc, face_index(s, q, axis)));
+}
+// coupled_motion.glslinc:112-137: staggered trilinear sample in fma difference form
+__device__ __noinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+  F3 lattice = mkf(x.x / s.h[0] - .5f, x.y / s.h[1] - .5f, x.z / s.h[2] - .5f);
+  comp(lattice, axis) += .5f;
+  I3 lo = mk((int)floorf(lattice.x), (int)floorf(lattice.y), (int)floorf(lattice.z));
+  F3 t = mkf(fract(lattice.x), fract(lattice.y), fract(lattice.z));
+  lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
+  bool interior = (s.flags & 8) != 0 && lo.x >= 1 && lo.y >= 1 && lo.z >= 1 && lo.x + 1 < s.nx && lo.y + 1 < s.ny && lo.z + 1 < s.nz &&
+                  !solid(s, mk(lo.x + 1, lo.y - 1, lo.z));
+  I3 fn = dims(s); comp(fn, axis)++;
+  int acc = accepted(s);
+  float values[8];
+#pragma unroll
+  for (int c8 = 0; c8 < 8; c8++) {
+    int xx = c8 & 1, y = (c8 >> 1) & 1, z = c8 >> 2;
+    bool supported; float value;
+    if (interior) {
+      int64_t f = face_local(s, lo, axis) + (xx + (int64_t)fn.x * (y + (int64_t)fn.y * z));
+      supported = s.motion[valid_base(s, acc) + f] > .5f;
+      value = forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
+    } else value = velocity_donor(s, mk(lo.x + xx, lo.y + y, lo.z + z), axis, forward, supported);
+    values[c8] = value;
+    float wx = (1 - t.x) * (1 - (float)xx) + t.x * (float)xx;
+    float wy = (1 - t.y) * (1 - (float)y) + t.y * (float)y;
+    float wz = (1 - t.z) * (1 - (float)z) + t.z * (float)z;
+    if (wx * wy * wz > 0) { valid = valid && supported; lim_lo = gmin(lim_lo, value); lim_hi = gmax(lim_hi, value); }
+  }
+  float a = fmaf(t.x, values[1] - values[0], values[0]), b = fmaf(t.x, values[3] - values[2], values[2]);
+  float d = fmaf(t.x, values[5] - values[4], values[4]), e = fmaf(t.x, values[7] - values[6], values[6]);
+  float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
+  return fmaf(t.z, upper - lower, lower);
+}
+__device__ __forceinline__ F3 sampled_velocity_checked(const Ctx& s, F3 x, bool& supported) {
+  F3 v; supported = true;
+  for (int axis = 0; axis < 3; axis++) { bool ok; float a, b; comp(v, axis) = sampled_component(s, x, axis, false, ok, a, b); supported = supported && ok; }
+  return v;
+}
+__device__ __forceinline__ float checked_endpoint(const Ctx& s, F3 x, int component, bool forward, bool& supported, float& l0, float& l1) {
+  float value = 0; supported = true; l0 = 0; l1 = 0;
+  for (int axis = 0; axis < 3; axis++) {
+    bool ok; float a, b; float dv = sampled_component(s, x, axis, axis == component && forward, ok, a, b);
+    supported = supported && ok; if (axis == component) { value = dv; l0 = a; l1 = b; }
+  }
+  return value;
+}
+__device__ __forceinline__ I3 cell_of(const Ctx& s, F3 p) { return mk((int)floorf(p.x / s.h[0]), (int)floorf(p.y / s.h[1]), (int)floorf(p.z / s.h[2])); }
+__device__ __forceinline__ bool trace_points_clear(const Ctx& s, F3 x, F3 mid, F3 end) {
+  return !solid(s, cell_of(s, x)) && !solid(s, cell_of(s, mid)) && !solid(s, cell_of(s, end));
+}
+__device__ __forceinline__ F3 trace_rk2(const Ctx& s, F3 x, float dt) {
+  bool ok; F3 u = sampled_velocity_checked(s, x, ok);
+  F3 mid = mkf(x.x - .5f * dt * u.x, x.y - .5f * dt * u.y, x.z - .5f * dt * u.z);
+  u = sampled_velocity_checked(s, mid, ok);
+  return mkf(x.x - dt * u.x, x.y - dt * u.y, x.z - dt * u.z);
+}
+__device__ __noinline__ bool trace_supported(const Ctx& s, F3 x, float dt) {
+  bool ok; F3 u = sampled_velocity_checked(s, x, ok);
+  F3 mid = mkf(x.x - .5f * dt * u.x, x.y - .5f * dt * u.y, x.z - .5f * dt * u.z);
+  u = sampled_velocity_checked(s, mid, ok);
+  F3 end = mkf(x.x - dt * u.x, x.y - dt * u.y, x.z - dt * u.z);
+  for (int stage = 0; stage < 3; stage++) {
+    F3 p = stage == 0 ? x : (stage == 1 ? mid : end);
+    if (solid(s, cell_of(s, p))) return false;
+    for (int axis = 0; axis < 3; axis++) { bool v; float a, b; sampled_component(s, p, axis, false, v, a, b); if (!v) return false; }
+  }
+  return true;
+}
+// coupled_motion.glslinc:189-202
+__device__ __forceinline__ float courant_at(const Ctx& s, I3 q, int bank, bool accelerate) {
+  F3 speed;
+  for (int axis = 0; axis < 3; axis++) {
+    I3 r = q; comp(r, axis)++;
+    comp(speed, axis) = gmax(fabsf(getv(s, bank, face_index(s, q, axis))), fabsf(getv(s, bank, face_index(s, r, axis))));
+  }
+  if (accelerate && coastal_rate(s, center(s, q).x) > 0) {
+    float se, ss; coastal_surface(s, center(s, q).x, s.c[1] + s.dt_max, se, ss);
+    if (center(s, q).y < se + 3 * s.h[1]) {
+      for (int axis = 0; axis < 3; axis++) {
+        I3 r = q; comp(r, axis)++;
+        comp(speed, axis) = gmax(compc(speed, axis), gmax(fabsf(coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1] + s.dt_max, axis)),
+                                                          fabsf(coastal_velocity_axis(s, coastal_face_position(s, r, axis), s.c[1] + s.dt_max, axis))));
+      }
+    }
+  }
+  if (accelerate) speed.y += 9.81f * s.dt_max * s.gmult;
+  speed.x /= s.h[0]; speed.y /= s.h[1]; speed.z /= s.h[2];
+  return s.max_face_cfl > .5f ? gmax(speed.x, gmax(speed.y, speed.z)) : dot3(speed, mkf(1, 1, 1));
+}
+__device__ __forceinline__ float linear_streamfunction(const Ctx& s, F3 x) {
+  float A = s.wave[0], k = s.wave[1], w = s.wave[2], h = s.wave[3];
+  float value = A * w / (k * sinhf(k * h)) * sinhf(k * (x.y + h)) * cosf(k * x.x);
+  if (s.stokes2 > .5f) value += .375f * A * A * w / powf(sinhf(k * h), 4.0f) * sinhf(2 * k * (x.y + h)) * cosf(2 * k * x.x);
+  return value;
+}
+__device__ __forceinline__ bool projected_face(const Ctx& s, I3 q, int axis) {
+  q = wrap_face(s, q); if (!face_in_bounds(s, q, axis) || wall_face(s, q, axis)) return false;
+  I3 l = q; comp(l, axis)--; return wet(s, q) || wet(s, l);
+}
+// coupled_motion.glslinc:212-230 (linear air continuation, numerics.y)
+__device__ __noinline__ float extended_value(const Ctx& s, int64_t source, I3 target) {
+  int cb = candidate(s);
+  float value = getv(s, cb, U_(s) + source);
+  if (s.linear_air < .5f) return value;
+  I3 q; int component; unpack_face(s, source, q, component);
+  F3 g = mkf(0, 0, 0);
+  for (int axis = 0; axis < 3; axis++) {
+    I3 l = q, r = q; comp(l, axis)--; comp(r, axis)++; l = wrap_face(s, l); r = wrap_face(s, r);
+    bool left = projected_face(s, l, component), right = projected_face(s, r, component);
+    float dl = left ? value - getv(s, cb, face_index(s, l, component)) : 0;
+    float dr = right ? getv(s, cb, face_index(s, r, component)) - value : 0;
+    if (left && right) comp(g, axis) = dl * dr > 0 ? gsign(dl) * gmin(fabsf(dl), fabsf(dr)) : 0;
+    else comp(g, axis) = left ? dl : (right ? dr : 0);
+  }
+  F3 off = periodic_delta(s, target, q);
+  off = mkf(gclamp(off.x / s.h[0], -1, 1), gclamp(off.y / s.h[1], -1, 1), gclamp(off.z / s.h[2], -1, 1));
+  return value + dot3(g, off);
+}

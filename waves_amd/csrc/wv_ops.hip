@@ -1,0 +1,777 @@
+// Per-step kernels of the gfx950 solver: initialisation, trial control, split
+// geometric VOF, limited MacCormack, coastal relaxation, geometry, PCG vector
+// ops, projection, extension and acceptance.  Each kernel restates one op (or
+// one op and its single-group finaliser, fused through a last-block ticket) of
+// water/shaders/compute/{pressure_pass,pressure_iterations,coupled_motion,
+// coastal_pass}.glslinc.  Reductions run in 128-lane groups with wavefront
+// shuffles so that their float sums match the reference's workgroup trees.
+#include "wv_motion.h"
+#include "wv_kernels.h"
+
+#define GID() ((int64_t)blockIdx.x * blockDim.x + threadIdx.x)
+
+// ---------------------------------------------------------------- init ops --
+// pressure_pass.glslinc:3-19 (op 0)
+__global__ void k_op0(Ctx s) {
+  int64_t i = GID();
+  if (i < s.stride) { s.bank[0][i] = 0; s.bank[1][i] = 0; }
+  if (i < WV_NCTRL) s.c[i] = 0;
+  if (i < s.N) {
+    I3 q = coord(i, dims(s)); float ph = center(s, q).y - eta(s, q);
+    bool wall = solid(s, q); s.geo[i] = make_float2(ph, wall ? -1.0f : (ph < 0.0f ? 1.0f : 0.0f));
+    float f = wall ? 0.0f : gclamp(0.5f - ph / s.h[1], 0.0f, 1.0f); s.bank[0][i] = f; s.bank[1][i] = f;
+  }
+}
+// coupled_motion.glslinc:234-270 (op 40): fixtures, dt, gravity multiplier
+__global__ void k_op40(Ctx s) {
+  int64_t i = GID(); int acc = accepted(s);
+  if (i < s.N) {
+    I3 q = coord(i, dims(s));
+    float lo[3] = {(float)q.x * s.h[0], (float)q.y * s.h[1], (float)q.z * s.h[2]};
+    float hi[3] = {lo[0] + s.h[0], lo[1] + s.h[1], lo[2] + s.h[2]};
+    float f = getv(s, acc, i);
+    if (s.shape > .5f && s.shape < 2.5f) {
+      float lower[3], upper[3];
+      if (s.shape < 1.5f) { lower[0] = 0; lower[1] = 0; lower[2] = 0; upper[0] = .28f; upper[1] = .3f; upper[2] = 100; }
+      else { lower[0] = .22f; lower[1] = .28f; lower[2] = .06f; upper[0] = .42f; upper[1] = .48f; upper[2] = .22f; }
+      f = 1; for (int k = 0; k < 3; k++) f *= gclamp((gmin(hi[k], upper[k]) - gmax(lo[k], lower[k])) / s.h[k], 0, 1);
+      if (s.shape > 1.5f) f += gclamp((.08f - lo[1]) / s.h[1], 0, 1);
+    }
+    if (s.shape == 4) {
+      f = 0;
+      for (int block = 0; block < 2; block++) {
+        float lower[3] = {block == 0 ? .08f : .36f, .12f, 0}, upper[3] = {block == 0 ? .24f : .52f, .32f, 100};
+        float o = 1; for (int k = 0; k < 3; k++) o *= gclamp((gmin(hi[k], upper[k]) - gmax(lo[k], lower[k])) / s.h[k], 0, 1);
+        f += o;
+      }
+    }
+    s.bank[0][i] = f; s.bank[1][i] = f;
+  }
+  if (i < s.NF) {
+    I3 q; int axis; unpack_face(s, i, q, axis);
+    float velocity = s.shape == 3 && axis == 0 && !wall_face(s, q, axis) && center(s, q).y < s.eta + 3 * s.h[1] ? .25f : 0;
+    if (s.shape == 4 && axis == 0 && !wall_face(s, q, axis)) velocity = (float)q.x * s.h[0] < .3f ? 2 : -2;
+    if (s.shape == 5) {
+      F3 x = mkf(((float)q.x + .5f) * s.h[0] + s.o[0], ((float)q.y + .5f) * s.h[1] + s.o[1], ((float)q.z + .5f) * s.h[2] + s.o[2]);
+      comp(x, axis) -= .5f * s.h[axis];
+      if (x.y < eta(s, q) + 3 * s.h[1] && !wall_face(s, q, axis)) {
+        if (axis == 0) { F3 p = x, m = x; p.y += .5f * s.h[1]; m.y -= .5f * s.h[1]; velocity = (linear_streamfunction(s, p) - linear_streamfunction(s, m)) / s.h[1]; }
+        if (axis == 1) { F3 p = x, m = x; p.x += .5f * s.h[0]; m.x -= .5f * s.h[0]; velocity = -(linear_streamfunction(s, p) - linear_streamfunction(s, m)) / s.h[0]; }
+      }
+    }
+    s.bank[0][U_(s) + i] = velocity; s.bank[1][U_(s) + i] = velocity;
+  }
+  if (i == 0) { s.c[20] = s.dt_max; s.c[31] = s.gmult; }
+}
+// Seeded alongshore perturbation of the still surface (benchmark input; no
+// reference op): F = clamp(1/2 - (y - eta - a*u(z))/dy), u(z) in [-1, 1).
+__device__ __forceinline__ uint32_t wv_hash32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16; return x;
+}
+__global__ void k_perturb(Ctx s) {
+  int64_t i = GID(); if (i >= s.N || s.pert_amp == 0) return;
+  I3 q = coord(i, dims(s)); if (solid(s, q)) return;
+  uint32_t hsh = wv_hash32(s.seed * 0x9e3779b9U + (uint32_t)q.z * 0x85ebca6bU + 0x632be5abU);
+  float u = (float)(hsh >> 8) * (1.0f / 8388608.0f) - 1.0f;
+  float ph = center(s, q).y - (eta(s, q) + s.pert_amp * u);
+  float f = gclamp(0.5f - ph / s.h[1], 0.0f, 1.0f); s.bank[0][i] = f; s.bank[1][i] = f;
+}
+
+// ------------------------------------------------------- geometry (17,18,16) --
+// MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
+template <int MODE>
+__global__ void k_reconstruct(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  if (MODE == 0 && s.c[4] > .5f && s.c[5] < .5f) return;
+  if (MODE == 1 && !(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (MODE == 2 && s.c[4] > .5f) return;
+  I3 q = coord(i, dims(s));
+  int gbank = geometry_bank(s, MODE == 2);
+  float f = MODE == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+  float4 e;
+  if (f <= 0 || f >= 1) e = make_float4(0, 1, 0, f);
+  else e = plane_from_gradient(s, q, interface_gradient(s, q, MODE == 1 ? 1 : 0, gbank), MODE == 1 ? 1 : 0, gbank, f);
+  if (MODE == 2) {
+    float fa = getv(s, accepted(s), i);
+    s.img_planes[i] = e; s.img_kinds[i] = fa <= 0 ? 0 : (fa >= 1 ? 1 : 2);
+  } else s.plane[i] = e;
+}
+// pressure_pass.glslinc:131-137 (op 18)
+__global__ void k_distance(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  if (s.c[4] > .5f && s.c[5] < .5f) return;
+  I3 q = coord(i, dims(s));
+  float ph = geometry_distance(s, q, geometry_bank(s, false));
+  s.geo[i] = make_float2(ph, solid(s, q) ? -1.0f : (ph < 0 ? 1.0f : 0.0f));
+}
+// pressure_pass.glslinc:108-115 (op 16)
+__global__ void k_coefficients(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  if (s.c[4] > .5f && s.c[5] < .5f) return;
+  I3 q = coord(i, dims(s)); float diag = 0; int face = 0; bool w = wet(s, q);
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    I3 nb = q; comp(nb, axis) += sign; float k = w ? coefficient(s, q, nb, axis) : 0;
+    s.work[(int64_t)(6 + face) * s.N + i] = k; diag += k; face++;
+  }
+  s.work[12 * s.N + i] = w ? diag : 1.0f;
+}
+// pressure_pass.glslinc:104-107 (op 14, plain-F export; not in the coupled program)
+__global__ void k_export_plain(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  float f = getv(s, accepted(s), i);
+  s.img_planes[i] = make_float4(0, 1, 0, f); s.img_kinds[i] = f == 0 ? 0 : (f == 1 ? 1 : 2);
+}
+
+// ------------------------------------------------------- trial control (43,1) --
+// coupled_motion.glslinc:351-355 then pressure_pass.glslinc:20-24
+__global__ void k_retry(Ctx s) {
+  float* c = s.c;
+  bool retry = c[7] == 20 || (c[7] >= 10 && c[7] <= 12);
+  if (retry && c[29] < 2) { c[33] = c[7]; c[34] = c[20]; c[30] = c[20] * .5f; c[29]++; c[7] = 0; c[4] = 0; c[5] = 0; }
+}
+__global__ void k_trial_start(Ctx s) {
+  float* c = s.c; float quota = *s.quota;
+  c[5] = 0;
+  if (c[7] > 0.5f || (quota >= 0 ? c[2] >= quota : c[1] >= -quota)) return;
+  if (c[4] < 0.5f) { c[4] = 1; c[5] = 1; c[6] = 0; c[8] = 0; c[13] = 1; c[32]++; c[57] = 0; c[58] = 0; }
+}
+__global__ void k_inject(Ctx s) { s.c[19] = 1; }  // op 15, fault injection
+
+// Serial in-order sum of the block partials' component k (the reference's lane-0
+// loops, e.g. pressure_pass.glslinc:80): the whole group stages the partials in
+// LDS and lane 0 adds them in index order.
+template <typename F>
+__device__ __forceinline__ void serial_partials(const float4* partial, int64_t groups, F&& fold) {
+  __shared__ float4 stage[512];
+  for (int64_t base = 0; base < groups; base += 512) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < 512 && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
+    __syncthreads();
+    if (threadIdx.x == 0) { int64_t n = groups - base < 512 ? groups - base : 512; for (int k = 0; k < n; k++) fold(stage[k]); }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------- CFL / dt (20 + 21) -----
+__global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID();
+  float rate = i < s.N ? courant_at(s, coord(i, dims(s)), accepted(s), true) : 0;
+  float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  // coupled_motion.glslinc:362-369: a max is order-free, so reduce it in parallel
+  float m = 0; int64_t G = gridDim.x;
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) m = gmax(m, s.partial[k].y);
+  float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
+  if (threadIdx.x == 0) {
+    float* c = s.c; float rt = r.x;
+    c[20] = gmin(s.dt_max, .45f / gmax(rt, 1e-20f)); if (c[29] > 0) c[20] = gmin(c[20], c[30]);
+    c[21] = 0; c[22] = 0; c[23] = 0; c[39] = 0;
+    for (int field = 64; field < 70; field++) c[field] = 0;
+  }
+}
+// ---------------------------------------------- compression audit (45 + 46) --
+__global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
+  if (i < s.N) {
+    int acc = accepted(s); float dt = timestep(s); float cell = s.h[0] * s.h[1] * s.h[2];
+    I3 q = coord(i, dims(s)); float f = getv(s, acc, i), dv = 0, raw = 0;
+    for (int axis = 0; axis < 3; axis++) {
+      I3 r = q; comp(r, axis)++;
+      dv += (old_face(s, r, axis) - old_face(s, q, axis)) / s.h[axis];
+      raw += (getv(s, acc, face_index(s, r, axis)) - getv(s, acc, face_index(s, q, axis))) / s.h[axis];
+    }
+    v = make_float4(f > .5f ? dt * dv * cell : 0, f > .5f && s.motion[pmask_base(s, acc) + i] < .5f ? 1 : 0, f * cell, f > .5f ? dt * raw * cell : 0);
+  }
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  if (threadIdx.x == 0) { float* c = s.c; c[35] = g.x; c[36] = g.y; c[37] = g.z; c[38] = g.w; if (g.y > 0) { c[7] = 22; c[3]++; } }
+}
+// ---------------------------------------------------------- bank copy (22) --
+__global__ void k_bank_copy(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); int acc = accepted(s), cb = 1 - acc;
+  if (i < s.stride) putv(s, cb, i, getv(s, acc, i));
+  if (i < s.N) s.motion[s.N + i] = getv(s, acc, i) > .5f ? 1 : 0;
+}
+// -------------------------------------------------- geometric flux (24) -----
+__global__ void k_flux(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  int sweep_axis = ((int)s.c[2] + (int)s.c[21]) % 3;
+  I3 q; int axis; unpack_face(s, i, q, axis);
+  float amount = 0;
+  if (axis == sweep_axis && !wall_face(s, q, axis)) {
+    float velocity = old_face(s, q, axis), courant = face_courant(s, q, axis);
+    I3 donor = q; if (velocity > 0) comp(donor, axis)--;
+    donor = wrap_cell(s, donor); float f = transport_f(s, donor), width = fabsf(courant); amount = f * width;
+    if (f > 0 && f < 1 && width > 0) {
+      float4 eq = donor.x < 0 ? coastal_plane(s, donor, s.c[1]) : s.plane[index3(donor, dims(s))];
+      F3 low = mkf(0, 0, 0), extent = mkf(1, 1, 1); comp(extent, axis) = width; if (velocity > 0) comp(low, axis) = 1 - width;
+      amount = width * cube_volume(mkf(eq.x * extent.x, eq.y * extent.y, eq.z * extent.z), eq.w - dot3(mkf(eq.x, eq.y, eq.z), low));
+    }
+    amount *= gsign(velocity);
+  }
+  s.motion[flux_base(s) + i] = amount;
+}
+// ---------------------------------------------- boundary ledgers (55 + 56) --
+__global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
+  if (i < s.N) {
+    int axis = ((int)s.c[2] + (int)s.c[21]) % 3;
+    I3 q = coord(i, dims(s));
+    if (axis == 1 && q.y == s.ny - 1) { q.y++; v.x = s.motion[flux_base(s) + face_local(s, q, 1)] * s.h[0] * s.h[1] * s.h[2]; }
+    if (axis == 0 && q.x == 0) { float flux = s.motion[flux_base(s) + face_local(s, q, 0)] * s.h[0] * s.h[1] * s.h[2]; v.y = gmax(flux, 0); v.z = gmax(-flux, 0); }
+  }
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  if (threadIdx.x == 0) { s.c[39] += g.x; s.c[65] += g.y; s.c[66] += g.z; }
+}
+// ------------------------------------------------ VOF update (25 + 26) ------
+__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); float4 v = make_float4(1e20f, -1e20f, 0, 0);
+  int sweep = (int)s.c[21];
+  if (i < s.N) {
+    int axis = ((int)s.c[2] + sweep) % 3;
+    I3 q = coord(i, dims(s)), r = q; comp(r, axis)++;
+    float net = s.motion[flux_base(s) + face_local(s, q, axis)] - s.motion[flux_base(s) + face_local(s, r, axis)];
+    float correction = s.motion[s.N + i] * (face_courant(s, r, axis) - face_courant(s, q, axis));
+    float increment = net + correction;
+    float f = transport_f(s, q) + increment; float change = 0;
+    if (f >= -1e-6f && f <= 1 + 1e-6f) change = gclamp(f, 0, 1) - f;
+    if (sweep == 1) s.motion[i] = f + change; else putv(s, candidate(s), i, f + change);
+    v = make_float4(f, f, change, fabsf(change)); if (isnan(f) || isinf(f)) { v.x = -1e30f; v.y = 1e30f; }
+  }
+  v = group_reduce<RED_BOUNDS>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  // coupled_motion.glslinc:402-408: min/max in parallel, the two sums in index order
+  float mn = 1e20f, mx = -1e20f;
+  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
+  float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
+  __shared__ float zs, ws;
+  if (threadIdx.x == 0) { zs = 0; ws = 0; }
+  serial_partials(s.partial, gridDim.x, [&](float4 p) { zs += p.z; ws += p.w; });
+  if (threadIdx.x == 0) {
+    float* c = s.c; float cell = s.h[0] * s.h[1] * s.h[2];
+    uint32_t step = (uint32_t)(int64_t)c[32] - 1u, sw = (uint32_t)c[21];
+    float4 hv = make_float4(b.x, b.y, zs, ws);
+    if (step < (uint32_t)s.hist_cap) s.history[(int64_t)step * 4 + sw] = hv;
+    c[22] += zs * cell; c[23] += ws * cell;
+    if (b.x < -1e-6f || b.y > 1 + 1e-6f) { c[7] = 10 + c[21]; c[3]++; }
+    c[21]++;
+  }
+}
+// ---------------------------------------------- MacCormack (27 / 28) --------
+// coupled_motion.glslinc:168-188, 409-426 (transport-cache and plain forms)
+template <bool CORRECT>
+__global__ void k_maccormack(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  I3 q; int axis; unpack_face(s, i, q, axis);
+  F3 x = mkf(((float)q.x + .5f) * s.h[0], ((float)q.y + .5f) * s.h[1], ((float)q.z + .5f) * s.h[2]);
+  comp(x, axis) -= .5f * s.h[axis];
+  int acc = accepted(s); float dt = timestep(s);
+  float value = 0;
+  if (!wall_face(s, q, axis) && ((s.flags & 2) == 0 || s.motion[valid_base(s, acc) + i] > .5f)) {
+    if (s.flags & 4) {
+      int64_t base = trace_base(s), nf = s.NF;
+      if (!CORRECT) {
+        bool vs, vm, ve; float l0, l1;
+        F3 sv = sampled_velocity_checked(s, x, vs);
+        F3 mid = mkf(x.x - .5f * dt * sv.x, x.y - .5f * dt * sv.y, x.z - .5f * dt * sv.z);
+        F3 um = sampled_velocity_checked(s, mid, vm);
+        F3 end = mkf(x.x - dt * um.x, x.y - dt * um.y, x.z - dt * um.z);
+        value = checked_endpoint(s, end, axis, false, ve, l0, l1);
+        s.motion[base + i] = l0; s.motion[base + nf + i] = l1;
+        s.motion[base + 2 * nf + i] = vs && vm && ve && trace_points_clear(s, x, mid, end) ? 1 : 0;
+        s.motion[base + 3 * nf + i] = sv.x; s.motion[base + 4 * nf + i] = sv.y; s.motion[base + 5 * nf + i] = sv.z;
+      } else {
+        float forward = s.motion[forward_base(s) + i];
+        if (s.motion[base + 2 * nf + i] < .5f) value = forward;
+        else {
+          F3 sv = mkf(s.motion[base + 3 * nf + i], s.motion[base + 4 * nf + i], s.motion[base + 5 * nf + i]);
+          bool vm, ve; float u0, u1; float rdt = -dt;
+          F3 mid = mkf(x.x - .5f * rdt * sv.x, x.y - .5f * rdt * sv.y, x.z - .5f * rdt * sv.z);
+          F3 um = sampled_velocity_checked(s, mid, vm);
+          F3 end = mkf(x.x - rdt * um.x, x.y - rdt * um.y, x.z - rdt * um.z);
+          float backward = checked_endpoint(s, end, axis, true, ve, u0, u1);
+          float corrected = forward + .5f * (getv(s, acc, U_(s) + i) - backward);
+          value = vm && ve && trace_points_clear(s, x, mid, end) && corrected >= s.motion[base + i] && corrected <= s.motion[base + nf + i] ? corrected : forward;
+        }
+      }
+    } else if (!CORRECT) {
+      bool ok; float a, b; value = sampled_component(s, trace_rk2(s, x, dt), axis, false, ok, a, b);
+    } else {
+      bool valid, vf; float a, b, d0, d1;
+      float backward = sampled_component(s, trace_rk2(s, x, -dt), axis, true, valid, a, b);
+      sampled_component(s, trace_rk2(s, x, dt), axis, false, vf, d0, d1);
+      float forward = s.motion[forward_base(s) + i], corrected = forward + .5f * (getv(s, acc, U_(s) + i) - backward);
+      value = valid && vf && trace_supported(s, x, dt) && trace_supported(s, x, -dt) && corrected >= d0 && corrected <= d1 ? corrected : forward;
+    }
+  }
+  if (!CORRECT) s.motion[forward_base(s) + i] = value; else putv(s, candidate(s), U_(s) + i, value);
+}
+
+// --------------------------------------------------- coastal relaxation -----
+__device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
+  F3 v;
+  for (int axis = 0; axis < 3; axis++) { I3 r = q; comp(r, axis)++; comp(v, axis) = .5f * (getv(s, bank, face_index(s, q, axis)) + getv(s, bank, face_index(s, r, axis))); }
+  return v;
+}
+__device__ __forceinline__ bool offshore_zone(const Ctx& s, float x) { return x < s.forc[0] + s.forc[1]; }
+// coastal_pass.glslinc:46-58 (ops 80 + 81)
+__global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
+  if (i < s.N) {
+    I3 q = coord(i, dims(s));
+    if (!solid(s, q)) {
+      int cb = candidate(s); float time = s.c[1] + timestep(s); float cell = s.h[0] * s.h[1] * s.h[2];
+      F3 cc = center(s, q);
+      float old = getv(s, cb, i), rate = coastal_rate(s, cc.x);
+      float target = offshore_zone(s, cc.x) ? coastal_fraction(s, q, time) : gclamp(.5f - cc.y / s.h[1], 0, 1);
+      float value = old + (1 - expf(-rate * timestep(s))) * (target - old); putv(s, cb, i, value);
+      F3 cv = cell_velocity(s, cb, q); float m = 1000 * (value - old) * cell;
+      v = make_float4((value - old) * cell, m * cv.x, m * cv.y, m * cv.z);
+    }
+  }
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  if (threadIdx.x == 0) { s.c[64] += g.x; s.c[67] += g.y; s.c[68] += g.z; s.c[69] += g.w; }
+}
+// coastal_pass.glslinc:59-68 (op 82)
+__global__ void k_relax_faces(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  I3 q; int axis; unpack_face(s, i, q, axis);
+  int cb = candidate(s); float time = s.c[1] + timestep(s);
+  F3 position = coastal_face_position(s, q, axis); float old = getv(s, cb, U_(s) + i), value = old;
+  if (!wall_face(s, q, axis)) {
+    float target = offshore_zone(s, position.x) ? coastal_velocity_axis(s, position, time, axis) : 0;
+    float t = 1 - expf(-coastal_rate(s, position.x) * timestep(s));
+    value = old * (1 - t) + target * t;
+    if (axis == 0 && q.x == 0) value = target;
+  }
+  putv(s, cb, U_(s) + i, value); s.motion[forward_base(s) + i] = value - old;
+}
+// coastal_pass.glslinc:69-77 (ops 83 + 84)
+__global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
+  if (i < s.N) {
+    I3 q = coord(i, dims(s)); F3 ch; float cell = s.h[0] * s.h[1] * s.h[2];
+    for (int axis = 0; axis < 3; axis++) { I3 r = q; comp(r, axis)++; comp(ch, axis) = .5f * (s.motion[forward_base(s) + face_local(s, q, axis)] + s.motion[forward_base(s) + face_local(s, r, axis)]); }
+    float m = getv(s, candidate(s), i) * 1000 * cell;
+    v = make_float4(ch.x * m, ch.y * m, ch.z * m, 0);
+  }
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  if (threadIdx.x == 0) { s.c[67] += g.x; s.c[68] += g.y; s.c[69] += g.z; }
+}
+// coastal_pass.glslinc:84-92 (ops 87 + 88)
+__global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, unsigned* ticket) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID();
+  float f = i < s.N ? getv(s, candidate(s), i) : 0;
+  float4 v = (isnan(f) || isinf(f)) ? make_float4(-1e30f, 1e30f, 0, 0) : make_float4(f, f, 0, 0);
+  v = group_reduce<RED_BOUNDS>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float mn = 1, mx = 0;
+  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
+  float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
+  if (threadIdx.x == 0) { s.c[88] = b.x; s.c[89] = b.y; if (b.x < 0 || b.y > 1) { s.c[7] = 23; s.c[3]++; } }
+}
+// coastal_pass.glslinc:78-83 (op 86): eight analytic target probes
+__global__ void k_probes(Ctx s) {
+  int i = threadIdx.x; if (i >= 8) return;
+  float probe_time = s.forc[3] * 6.28318530718f / s.wave[2] * (i < 4 ? .37f : 1);
+  F3 point = mkf(s.o[0] + .17f + (float)i * .23f, -s.wave[3] + (float)i * .07f, 0);
+  float se, ss; coastal_surface(s, point.x, probe_time, se, ss); F3 vel = coastal_velocity(s, point, probe_time);
+  s.probe[i] = make_float4(se, ss, vel.x, vel.y);
+}
+// coastal_pass.glslinc:102-113 (op 85): eight column-height gauges; one block
+// per gauge, columns in parallel over z, the z sum in index order
+__global__ void k_gauges(Ctx s) {
+  __shared__ float columns[1024];
+  int g = blockIdx.x; int acc = accepted(s);
+  float begin = s.forc[0] + s.forc[1] + .4f;
+  float end = s.o[0] + (float)s.nx * s.h[0] - (s.shape == 6 ? s.forc[1] + .4f : 1.0f);
+  float t = (float)g / 7; float m = begin * (1 - t) + end * t;
+  int x = iclamp((int)((m - s.o[0]) / s.h[0]), 0, s.nx - 1);
+  float height = 0;
+  for (int z0 = 0; z0 < s.nz; z0 += 1024) {
+    __syncthreads();
+    for (int z = z0 + threadIdx.x; z < s.nz && z < z0 + 1024; z += blockDim.x) {
+      float column = s.o[1];
+      for (int y = 0; y < s.ny; y++) { I3 q = mk(x, y, z); column += (solid(s, q) ? 1 : getv(s, acc, index3(q, dims(s)))) * s.h[1]; }
+      columns[z - z0] = column;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) for (int z = z0; z < s.nz && z < z0 + 1024; z++) height += columns[z - z0];
+  }
+  if (threadIdx.x == 0) s.c[80 + g] = height / (float)s.nz;
+}
+// coastal_pass.glslinc:93-101 (op 89)
+__global__ void k_coastal_history(Ctx s) {
+  float* c = s.c;
+  if (c[2] <= c[91] || c[2] > s.hist_cap) return;
+  int64_t row = ((int64_t)c[2] - 1) * 5;
+  s.chist[row] = make_float4(c[1], c[90], c[17], c[72]);
+  s.chist[row + 1] = make_float4(c[73], c[74], c[40], c[24]);
+  s.chist[row + 2] = make_float4(c[80], c[81], c[82], c[83]);
+  s.chist[row + 3] = make_float4(c[84], c[85], c[86], c[87]);
+  s.chist[row + 4] = make_float4(c[75], c[76], c[77], c[28]);
+  c[91] = c[2];
+}
+// coupled_motion.glslinc:338-340 (op 44)
+__global__ void k_history(Ctx s) {
+  float* c = s.c;
+  if (c[32] > 0 && c[32] <= s.hist_cap) {
+    int64_t row = ((int64_t)c[32] - 1) * 4 + 3;
+    s.history[row] = make_float4(c[1], c[20], c[7] > 0 ? -c[7] : (c[4] < .5f ? 1 : 0), c[2]);
+  }
+}
+
+// ------------------------------------------- pressure preparation (2, 47) ---
+// pressure_pass.glslinc:25-50 (op 2)
+__global__ void k_gravity_prep(Ctx s) {
+  if (s.c[5] < 0.5f) return;
+  int64_t i = GID(); if (i >= s.stride) return;
+  int acc = accepted(s), cb = 1 - acc; float dt = timestep(s);
+  int64_t P = P_(s), Ub = U_(s), Vb = V_(s), Wb = W_(s);
+  float val = getv(s, cb, i);
+  if (i < s.N) s.work[13 * s.N + i] = 0;
+  if (i >= P) val = wet(s, coord(i - P, dims(s))) ? getv(s, acc, i) : 0;
+  if (i >= Ub && i < P) {
+    int axis = i < Vb ? 0 : (i < Wb ? 1 : 2); int64_t base = axis == 0 ? Ub : (axis == 1 ? Vb : Wb);
+    I3 fn = dims(s); comp(fn, axis)++;
+    I3 r = coord(i - base, fn), l = r; comp(l, axis)--;
+    if (solid(s, l) || solid(s, r) || (!wet(s, l) && !wet(s, r))) val = 0.0f;
+    else if (axis == 1) val -= 9.81f * dt * s.c[31];
+  }
+  putv(s, cb, i, val);
+}
+// coupled_motion.glslinc:319-322 (op 47)
+__global__ void k_periodic_sync(Ctx s) {
+  if (s.c[4] < .5f || s.c[7] > .5f) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  I3 q; int axis; unpack_face(s, i, q, axis);
+  int64_t canonical = face_index(s, q, axis);
+  if (canonical != U_(s) + i) { int cb = candidate(s); putv(s, cb, U_(s) + i, getv(s, cb, canonical)); }
+}
+// coupled_motion.glslinc:282-284 (ops 63, 64, 65)
+__global__ void k_defect_state(Ctx s) { float* c = s.c; if (c[57] == 1 && c[7] < .5f) { c[57] = 2; c[6] = 0; c[8] = 0; } }
+__global__ void k_defect_save(Ctx s) {
+  if (!(s.c[57] == 2 && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.N) return;
+  int cb = candidate(s); s.work[13 * s.N + i] = getv(s, cb, P_(s) + i); putv(s, cb, P_(s) + i, 0);
+}
+__global__ void k_defect_restore(Ctx s) {
+  if (!(s.c[4] > .5f && s.c[6] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.N) return;
+  int cb = candidate(s); putv(s, cb, P_(s) + i, getv(s, cb, P_(s) + i) + s.work[13 * s.N + i]);
+}
+
+// --------------------------------------------------------------- PCG ops ----
+__device__ __forceinline__ float vector_at(const Ctx& s, I3 q, int which) {
+  q = wrap_cell(s, q); if (!wet(s, q)) return 0.0f;
+  int64_t i = index3(q, dims(s));
+  return which == 0 ? press(s, q) : s.work[2 * s.N + i];
+}
+// pressure_core.glslinc:120-125: masked 7-point operator
+__device__ __forceinline__ float operator_at(const Ctx& s, I3 q, int which) {
+  float result = 0, here = vector_at(s, q, which); int64_t i = index3(q, dims(s));
+  int face = 0;
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    I3 nb = q; comp(nb, axis) += sign; float k = s.work[(int64_t)(6 + face) * s.N + i];
+    result += k * (here - vector_at(s, nb, which)); face++;
+  }
+  return result;
+}
+// pressure_iterations.glslinc:4-13 (op 3).  The block partials of op 3 are
+// replaced by the V-cycle's output partials before op 4 reads them
+// (pressure_multigrid.glslinc:230-234), so this kernel publishes none.
+__global__ void k_pcg_init(Ctx s) {
+  if (!pressure_setup(s)) return;
+  int64_t i = GID(); if (i >= s.N) return;
+  I3 q = coord(i, dims(s)); float r = 0, z = 0, bv = 0, diag = 1; float dt = timestep(s);
+  if (wet(s, q)) { diag = s.work[12 * s.N + i]; bv = -1000.0f / dt * divergence(s, q); r = bv - operator_at(s, q, 0); z = r / diag; }
+  int64_t N = s.N;
+  s.work[i] = r; s.work[N + i] = z; s.work[2 * N + i] = z; s.work[4 * N + i] = diag; s.work[5 * N + i] = bv;
+}
+// pressure_iterations.glslinc:20 (op 5)
+__global__ void k_spmv(Ctx s) {
+  if (!solving(s)) return;
+  int64_t i = GID(); if (i >= s.N) return;
+  I3 q = coord(i, dims(s)); s.work[3 * s.N + i] = wet(s, q) ? operator_at(s, q, 1) : 0;
+}
+// pressure_iterations.glslinc:21-22 (ops 6 + 7)
+__global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, unsigned* ticket) {
+  if (!solving(s)) return;
+  int64_t i = GID(); int64_t N = s.N;
+  float4 v = i < N ? make_float4(s.work[2 * N + i] * s.work[3 * N + i], 0, 0, 0) : make_float4(0, 0, 0, 0);
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float dotp = global_reduce<false>(s.partial, gridDim.x).x;
+  if (threadIdx.x == 0) {
+    if (dotp <= 0 || isnan(dotp) || isinf(dotp)) { s.c[7] = 1; return; }
+    s.c[11] = s.c[9] / dotp;
+  }
+}
+// pressure_iterations.glslinc:23-30 (op 8) fused with the V-cycle's load of r
+// (pressure_multigrid.glslinc:191).  z = r/diag and the op-8 partials are
+// superseded by the V-cycle output and are not written.
+__global__ void k_pcg_update(Ctx s, float* mg_b, float* mg_x1, float* mg_x2) {
+  if (!solving(s)) return;
+  int64_t i = GID(); int64_t N = s.N; if (i >= N) return;
+  float alpha = s.c[11];
+  float r = s.work[i] - alpha * s.work[3 * N + i];
+  int cb = candidate(s);
+  putv(s, cb, P_(s) + i, getv(s, cb, P_(s) + i) + alpha * s.work[2 * N + i]);
+  s.work[i] = r;
+  mg_b[i] = r; mg_x1[i] = 0; mg_x2[i] = 0;
+}
+// pressure_iterations.glslinc:32 (op 10)
+__global__ void k_direction(Ctx s) {
+  if (!solving(s)) return;
+  int64_t i = GID(); int64_t N = s.N; if (i >= N) return;
+  s.work[2 * N + i] = s.work[N + i] + s.c[12] * s.work[2 * N + i];
+}
+
+// --------------------------------------------------------- projection (11) --
+__global__ void k_project(Ctx s) {
+  if (s.c[4] < 0.5f || s.c[6] < 0.5f || s.c[7] > 0.5f) return;
+  int64_t i = U_(s) + GID(); int64_t P = P_(s); if (i >= P) return;
+  int64_t Vb = V_(s), Wb = W_(s);
+  int axis = i < Vb ? 0 : (i < Wb ? 1 : 2); int64_t base = axis == 0 ? U_(s) : (axis == 1 ? Vb : Wb);
+  I3 fn = dims(s); comp(fn, axis)++;
+  I3 r = coord(i - base, fn), l = r; comp(l, axis)--;
+  int cb = candidate(s); float dt = timestep(s);
+  float value = 0;
+  if (!solid(s, l) && !solid(s, r) && (wet(s, l) || wet(s, r)))
+    value = getv(s, cb, i) - dt / 1000.0f * (press(s, r) - press(s, l)) / distance_face(s, l, r, axis);
+  if (axis == 0 && r.x == 0 && !solid(s, r)) value = getv(s, cb, i);
+  putv(s, cb, i, value);
+}
+// ---------------------------------------------------- extension (30 - 34) --
+__device__ __forceinline__ bool post_ok(const Ctx& s) { return !(s.c[4] < .5f || s.c[6] < .5f || s.c[7] > .5f); }
+__global__ void k_ext_seed(Ctx s) {
+  if (!post_ok(s)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+  s.motion[source_base(s, 0) + i] = !wall_face(s, q, axis) && (wet(s, q) || wet(s, l)) ? (float)i : -1;
+}
+__global__ void k_ext_hop(Ctx s, int rb) {
+  if (!post_ok(s)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  int wb = 1 - rb;
+  I3 q; int axis; unpack_face(s, i, q, axis);
+  float source = s.motion[source_base(s, rb) + i], best = 1e20f;
+  if (!wall_face(s, q, axis)) {
+    if (source >= 0) { I3 sq; int sa; unpack_face(s, (int64_t)source, sq, sa); F3 d = periodic_delta(s, q, sq); best = dot3(d, d); }
+    for (int ax = 0; ax < 3; ax++) for (int sign = -1; sign <= 1; sign += 2) {
+      I3 nb = q; comp(nb, ax) += sign; nb = wrap_face(s, nb);
+      if (!face_in_bounds(s, nb, axis) || wall_face(s, nb, axis)) continue;
+      float cs = s.motion[source_base(s, rb) + face_local(s, nb, axis)]; if (cs < 0) continue;
+      I3 sq; int sa; unpack_face(s, (int64_t)cs, sq, sa); F3 d = periodic_delta(s, q, sq); float distance = dot3(d, d);
+      if (distance < best || (distance == best && cs < source)) { best = distance; source = cs; }
+    }
+  } else source = -1;
+  s.motion[source_base(s, wb) + i] = source;
+}
+__global__ void k_ext_fill(Ctx s) {
+  if (!post_ok(s)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+  if (wet(s, q) || wet(s, l)) return;
+  float source = s.motion[source_base(s, 1) + i];
+  float value = source < 0 ? 0 : extended_value(s, (int64_t)source, q);
+  putv(s, candidate(s), U_(s) + i, value);
+}
+// coupled_motion.glslinc:323-337 (ops 48 / 49)
+template <bool INITIAL>
+__global__ void k_masks(Ctx s) {
+  if (!INITIAL && !post_ok(s)) return;
+  int64_t i = GID(); int cb = candidate(s);
+  if (i < s.N) {
+    float flag = wet(s, coord(i, dims(s))) ? 1 : 0;
+    if (INITIAL) { s.motion[pmask_base(s, 0) + i] = flag; s.motion[pmask_base(s, 1) + i] = flag; }
+    else s.motion[pmask_base(s, cb) + i] = flag;
+  }
+  if (i < s.NF) {
+    I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+    float valid = INITIAL ? (!wall_face(s, q, axis) && (wet(s, q) || wet(s, l) || center(s, q).y < eta(s, q) + 3 * s.h[1]) ? 1 : 0)
+                          : (s.motion[source_base(s, 1) + i] >= 0 ? 1 : 0);
+    if (INITIAL) { s.motion[valid_base(s, 0) + i] = valid; s.motion[valid_base(s, 1) + i] = valid; }
+    else s.motion[valid_base(s, cb) + i] = valid;
+  }
+}
+// --------------------------------------------- post-projection CFL (35+36) --
+__global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, unsigned* ticket) {
+  if (!post_ok(s)) return;
+  int64_t i = GID();
+  float rate = i < s.N ? courant_at(s, coord(i, dims(s)), candidate(s), false) * timestep(s) : 0;
+  float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float m = 0;
+  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) m = gmax(m, s.partial[k].y);
+  float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
+  if (threadIdx.x == 0) s.c[27] = r.x;
+}
+// ------------------------------------------------ true residual (62 + 66) --
+__global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, unsigned* ticket) {
+  if (!post_ok(s)) return;
+  int64_t i = GID(); float r = 0;
+  if (i < s.N) { I3 q = coord(i, dims(s)); if (wet(s, q)) r = 1000 / timestep(s) * divergence(s, q); }
+  float4 v = group_reduce<RED_SUM>(make_float4(r * r, 0, 0, 0));
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float r2 = global_reduce<false>(s.partial, gridDim.x).x;
+  if (threadIdx.x == 0) {
+    float* c = s.c;
+    c[60] = sqrtf(r2 / gmax(c[59], 1e-30f)); c[62] = pressure_scaled_norm(s, c[59]); c[63] = pressure_scaled_norm(s, r2);
+    c[61] = pressure_norm_converged(s, r2, c[59]) ? 1 : 0;
+  }
+}
+// -------------------------------------- diagnostics + acceptance (12 + 13) --
+__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, unsigned* ticket) {
+  if (!post_ok(s)) return;
+  int64_t i = GID(); float4 value = make_float4(0, 0, 0, 0); float dt = timestep(s);
+  if (i < s.N) {
+    int cb = candidate(s); I3 q = coord(i, dims(s)); float speed = 0;
+    for (int axis = 0; axis < 3; axis++) {
+      I3 r = q; comp(r, axis)++;
+      speed = gmax(speed, gmax(fabsf(getv(s, cb, face_index(s, q, axis))), fabsf(getv(s, cb, face_index(s, r, axis)))));
+    }
+    bool w = wet(s, q);
+    float div = w ? dt * fabsf(divergence(s, q)) : 0;
+    float err = w ? fabsf(press(s, q) - 9810.0f * (eta(s, q) - center(s, q).y)) : 0;
+    value = make_float4(div, speed, err, getv(s, cb, i) * s.h[0] * s.h[1] * s.h[2]);
+    if (isnan(value.x) || isnan(value.y) || isnan(value.z) || isnan(value.w) || isinf(value.x) || isinf(value.y) || isinf(value.z) || isinf(value.w))
+      value = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
+  }
+  value = group_reduce<RED_DIAG>(value);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = value;
+  if (!last_block(ticket)) return;
+  // pressure_pass.glslinc:78-103: maxima in parallel, the volume in index order
+  float4 mx = make_float4(0, 0, 0, 0);
+  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
+  mx = group_reduce<RED_DIAG>(mx);
+  __shared__ float vol;
+  if (threadIdx.x == 0) vol = 0;
+  serial_partials(s.partial, gridDim.x, [&](float4 p) { vol += p.w; });
+  if (threadIdx.x != 0) return;
+  float* c = s.c; float4 v = make_float4(mx.x, mx.y, mx.z, vol);
+  c[14] = v.x; c[15] = v.y; c[16] = v.z; c[17] = v.w;
+  if ((v.x >= 1e-4f || c[61] < .5f) && v.y < 1e20f && c[19] < .5f && c[58] < 4) { c[57] = 1; c[58]++; c[6] = 0; return; }
+  if (c[61] < .5f) { c[3]++; c[7] = 4; return; }
+  if (v.x >= 1e-4f || v.y >= 1e20f || c[19] > 0.5f) { c[3]++; c[7] = 4; return; }
+  c[28] = v.w + c[40] + c[39] - c[26] - c[24] - c[22];
+  c[28] += c[74] + c[66] - c[73] - c[65] - c[72] - c[64];
+  if (fabsf(c[28]) > 1e-3f * gmax(c[26], 1e-20f)) { c[3]++; c[7] = 21; return; }
+  if (c[27] > .450001f) { c[3]++; c[7] = 20; return; }
+  c[24] += c[22]; c[25] += c[23]; c[40] += c[39]; c[29] = 0; c[30] = 0;
+  for (int field = 0; field < 6; field++) c[72 + field] += c[64 + field];
+  c[90] = dt;
+  c[0] = (float)candidate(s); c[1] += dt; c[2]++; c[4] = 0;
+}
+// ------------------------------------------------ initial volume (41 + 42) --
+__global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, unsigned* ticket) {
+  int64_t i = GID();
+  float4 v = i < s.N ? make_float4(getv(s, accepted(s), i) * s.h[0] * s.h[1] * s.h[2], 0, 0, 0) : make_float4(0, 0, 0, 0);
+  v = group_reduce<RED_SUM>(v);
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (!last_block(ticket)) return;
+  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  if (threadIdx.x == 0) s.c[26] = g.x;
+}
+
+// ------------------------------------------------------------- launchers ----
+static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+#define L256(k, n) hipLaunchKernelGGL(k, dim3(blocks(n, 256)), dim3(256), 0, st, s)
+#define LRED(k, id) hipLaunchKernelGGL(k, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, s.counters + (id))
+#define L1(k) hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, st, s)
+
+int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
+  switch (op) {
+  case 0: L256(k_op0, s.stride); break;
+  case 40: L256(k_op40, s.stride); break;
+  case 90: L256(k_perturb, s.N); break;
+  case 1: hipLaunchKernelGGL(k_trial_start, dim3(1), dim3(1), 0, st, s); break;
+  case 43: hipLaunchKernelGGL(k_retry, dim3(1), dim3(1), 0, st, s); break;
+  case 15: hipLaunchKernelGGL(k_inject, dim3(1), dim3(1), 0, st, s); break;
+  case 2: L256(k_gravity_prep, s.stride); break;
+  case 3: L256(k_pcg_init, s.N); break;
+  case 5: L256(k_spmv, s.N); break;
+  case 6: LRED(k_dot_alpha, 1); break;
+  case 10: L256(k_direction, s.N); break;
+  case 11: L256(k_project, s.NF); break;
+  case 12: LRED(k_accept, 2); break;
+  case 14: L256(k_export_plain, s.N); break;
+  case 16: L256(k_coefficients, s.N); break;
+  case 17: L256(k_reconstruct<0>, s.N); break;
+  case 18: L256(k_distance, s.N); break;
+  case 19: L256(k_reconstruct<2>, s.N); break;
+  case 20: LRED(k_cfl, 3); break;
+  case 22: L256(k_bank_copy, s.stride); break;
+  case 23: L256(k_reconstruct<1>, s.N); break;
+  case 24: L256(k_flux, s.NF); break;
+  case 25: LRED(k_vof_update, 4); break;
+  case 27: L256(k_maccormack<false>, s.NF); break;
+  case 28: L256(k_maccormack<true>, s.NF); break;
+  case 30: L256(k_ext_seed, s.NF); break;
+  case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 256)), dim3(256), 0, st, s, (op - 31) % 2); break;
+  case 34: L256(k_ext_fill, s.NF); break;
+  case 35: LRED(k_post_courant, 5); break;
+  case 41: LRED(k_volume, 6); break;
+  case 44: hipLaunchKernelGGL(k_history, dim3(1), dim3(1), 0, st, s); break;
+  case 45: LRED(k_compression, 7); break;
+  case 47: L256(k_periodic_sync, s.NF); break;
+  case 48: L256(k_masks<false>, s.NF > s.N ? s.NF : s.N); break;
+  case 49: L256(k_masks<true>, s.NF > s.N ? s.NF : s.N); break;
+  case 55: LRED(k_ledgers, 8); break;
+  case 62: LRED(k_true_residual, 9); break;
+  case 63: hipLaunchKernelGGL(k_defect_state, dim3(1), dim3(1), 0, st, s); break;
+  case 64: L256(k_defect_save, s.N); break;
+  case 65: L256(k_defect_restore, s.N); break;
+  case 80: LRED(k_relax_f, 10); break;
+  case 82: L256(k_relax_faces, s.NF); break;
+  case 83: LRED(k_momentum, 11); break;
+  case 85: hipLaunchKernelGGL(k_gauges, dim3(8), dim3(256), 0, st, s); break;
+  case 86: hipLaunchKernelGGL(k_probes, dim3(1), dim3(64), 0, st, s); break;
+  case 87: LRED(k_forcing_bounds, 12); break;
+  case 89: hipLaunchKernelGGL(k_coastal_history, dim3(1), dim3(1), 0, st, s); break;
+  // finalisers fused into the kernel of the preceding op, and PCG reductions
+  // fused into the V-cycle output kernel (op 4 and op 9)
+  case 7: case 13: case 21: case 26: case 36: case 42: case 46: case 56: case 66: case 81: case 84: case 88:
+  case 4: case 9: break;
+  case 8: return -2;  // needs the MG level-0 vectors: launched through wv_launch_pcg_update
+  default: return -1;
+  }
+  return 0;
+}
+void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, b0, x1, x2);
+}
+// masked 7-point SpMV launched on its own for live timing (writes only Ap)
+__global__ void k_spmv_timed(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  I3 q = coord(i, dims(s)); s.work[3 * s.N + i] = wet(s, q) ? operator_at(s, q, 1) : 0;
+}
+void wv_launch_spmv_timed(const Ctx& s, hipStream_t st) { L256(k_spmv_timed, s.N); }

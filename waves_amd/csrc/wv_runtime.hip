@@ -1,0 +1,478 @@
+// Host runtime and C-ABI of the gfx950 solver (libwaves_hip.so).
+//
+// Replaces the reference's render-thread runtime: buffer allocation
+// (water/pressure_gpu.gd:82-127, water/coupled_gpu.gd:23-33), multigrid
+// planning (water/pressure_multigrid.gd:33-134), the per-dispatch driver
+// (water/pressure_gpu.gd:69-81) and the chunk scheduler with one asynchronous
+// control readback per submission (water/work_scheduler.gd:41-87).
+// A batch of the coupled step program is captured once into a hipGraph and
+// replayed; every op is gated on the device-resident control vector, so the
+// host never decides acceptance, retries or PCG convergence.
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "wv_kernels.h"
+#include "../../include/waves.h"
+
+namespace {
+
+struct Offset { int x, y, z; };
+
+struct Plan {
+  int L = 0;
+  MGLevel lv[WV_MAXL];
+  std::vector<int> meta;            // 16 ints per level, the reference layout
+  std::vector<int> offsets;         // 3 ints per stencil entry
+  std::vector<signed char> slots;   // 125 per level
+  int64_t node_count = 0, row_count = 0;
+};
+
+// water/pressure_multigrid.gd:181-188
+static int basis(Offset q, const int f[3], bool aggregate, Offset* out) {
+  if (aggregate) { out[0] = {(int)std::floor((float)q.x / f[0]), (int)std::floor((float)q.y / f[1]), (int)std::floor((float)q.z / f[2])}; return 1; }
+  int b[3], qa[3] = {q.x, q.y, q.z};
+  for (int k = 0; k < 3; k++) b[k] = (int)std::floor(((double)qa[k] + .5) / f[k] - .5);
+  int m = 0;
+  for (int z = 0; z < f[2]; z++) for (int y = 0; y < f[1]; y++) for (int x = 0; x < f[0]; x++) out[m++] = {b[0] + x, b[1] + y, b[2] + z};
+  return m;
+}
+// water/pressure_multigrid.gd:189-198 (insertion order kept: it fixes the row layout)
+static std::vector<Offset> coarse_stencil(const std::vector<Offset>& st, const int f[3], bool aggregate) {
+  std::vector<Offset> out;
+  for (int z = 0; z < f[2]; z++) for (int y = 0; y < f[1]; y++) for (int x = 0; x < f[0]; x++) {
+    Offset q{x, y, z}, left[8]; int nl = basis(q, f, aggregate, left);
+    for (const Offset& o : st) for (int a = 0; a < nl; a++) {
+      Offset right[8]; int nr = basis({q.x + o.x, q.y + o.y, q.z + o.z}, f, aggregate, right);
+      for (int b = 0; b < nr; b++) {
+        Offset d{right[b].x - left[a].x, right[b].y - left[a].y, right[b].z - left[a].z};
+        bool found = false; for (const Offset& e : out) if (e.x == d.x && e.y == d.y && e.z == d.z) { found = true; break; }
+        if (!found) out.push_back(d);
+      }
+    }
+  }
+  return out;
+}
+// water/pressure_multigrid.gd:40-69
+static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
+  int n[3] = {(int)pc[0], (int)pc[1], (int)pc[2]}; float h[3] = {pc[4], pc[5], pc[6]};
+  bool perx = pc[12] > .5f;
+  std::vector<Offset> st = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  std::vector<std::vector<Offset>> stencils;
+  int64_t node_count = 0, matrix_count = 0;
+  for (;;) {
+    int count = n[0] * n[1] * n[2]; int f[3] = {1, 1, 1};
+    if (count > 64) {
+      float smallest = INFINITY;
+      for (int a = 0; a < 3; a++) if (n[a] > 2) smallest = std::fmin(smallest, h[a]);
+      for (int a = 0; a < 3; a++) if (n[a] > 2 && h[a] < 1.8f * smallest) f[a] = 2;
+      if (perx && n[0] % 2) f[0] = 1;
+      if (n[2] % 2) f[2] = 1;
+    }
+    if (p.L >= WV_MAXL) return false;
+    bool aggregate = p.L >= aggregate_from;
+    MGLevel& L = p.lv[p.L];
+    for (int a = 0; a < 3; a++) { L.n[a] = n[a]; L.f[a] = f[a]; }
+    L.count = count; L.aggregate = aggregate; L.width = (int)st.size();
+    L.vec_off = 8 * node_count;
+    int m[16] = {n[0], n[1], n[2], (int)node_count, (int)matrix_count, f[0], f[1], f[2], count, 0, 0, 0, 0, 0, 0, aggregate ? 1 : 0};
+    p.meta.insert(p.meta.end(), m, m + 16);
+    stencils.push_back(st);
+    p.L++;
+    node_count += count;
+    if (p.L > 1) matrix_count += (int64_t)count * 125;
+    if (count <= 64 || (f[0] == 1 && f[1] == 1 && f[2] == 1)) break;
+    st = coarse_stencil(st, f, aggregate);
+    for (int a = 0; a < 3; a++) { n[a] = (n[a] + f[a] - 1) / f[a]; h[a] *= (float)f[a]; }
+  }
+  int64_t row_count = 0, transfer_count = 0; int pos = 16 * p.L;
+  p.slots.assign(125 * p.L, (signed char)-1);
+  for (int l = 0; l < p.L; l++) {
+    MGLevel& L = p.lv[l]; int* m = &p.meta[16 * l];
+    m[10] = pos; m[11] = L.width; m[12] = (int)row_count; m[13] = (int)transfer_count;
+    m[14] = L.aggregate ? 1 : L.f[0] * L.f[1] * L.f[2];
+    L.off_start = (int)(p.offsets.size() / 3);
+    L.row_off = l > 0 ? row_count : 0;
+    if (l + 1 < p.L) transfer_count += (int64_t)L.count * m[14];
+    if (l > 0) row_count += (int64_t)L.count * L.width;
+    for (int k = 0; k < L.width; k++) {
+      const Offset& o = stencils[l][k];
+      p.offsets.push_back(o.x); p.offsets.push_back(o.y); p.offsets.push_back(o.z);
+      if (std::abs(o.x) <= 2 && std::abs(o.y) <= 2 && std::abs(o.z) <= 2) p.slots[125 * l + (o.x + 2 + 5 * (o.y + 2 + 5 * (o.z + 2)))] = (signed char)k;
+      pos += 3;
+    }
+  }
+  p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
+  p.node_count = node_count; p.row_count = row_count;
+  return true;
+}
+
+}  // namespace
+
+struct wv_solver {
+  wv_settings settings;
+  Ctx ctx;
+  MGDev mg;
+  Plan plan;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocations;
+  float* quota_dev = nullptr;
+  float* quota_host = nullptr;   // pinned
+  float* ctrl_host = nullptr;    // pinned, readback slots
+  hipEvent_t ctrl_event = nullptr;
+  std::atomic<double> quota{0.0};
+  float last_uploaded_quota = NAN;
+  bool readback_pending = false;
+  float last_control[WV_NCTRL] = {0};
+  hipGraphExec_t batch_exec = nullptr;
+  hipGraph_t batch_graph = nullptr;
+  int smoothing = 3, iterations = 8;
+  int64_t history_capacity = 0;
+  std::string error;
+  std::mutex lock;
+};
+
+static int fail(wv_solver* s, int code, const std::string& msg) {
+  if (s) s->error = msg;
+  return code;
+}
+#define HIPCHECK(s, expr)                                                           \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) return fail((s), WV_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dalloc(wv_solver* s, T** ptr, size_t count) {
+  void* p = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(&p, bytes) != hipSuccess) return fail(s, WV_ENOMEM, "hipMalloc failed for " + std::to_string(bytes) + " bytes");
+  if (hipMemset(p, 0, bytes) != hipSuccess) return fail(s, WV_EHIP, "hipMemset failed");
+  s->allocations.push_back(p);
+  *ptr = (T*)p;
+  return 0;
+}
+
+static void release(wv_solver* s) {
+  if (s->batch_exec) hipGraphExecDestroy(s->batch_exec);
+  if (s->batch_graph) hipGraphDestroy(s->batch_graph);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (auto it = s->allocations.rbegin(); it != s->allocations.rend(); ++it) hipFree(*it);
+  s->allocations.clear();
+  if (s->quota_host) hipHostFree(s->quota_host);
+  if (s->ctrl_host) hipHostFree(s->ctrl_host);
+  if (s->ctrl_event) hipEventDestroy(s->ctrl_event);
+  if (s->stream) hipStreamDestroy(s->stream);
+}
+
+extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
+  if (!settings || !out) return WV_EINVAL;
+  *out = nullptr;
+  const float* pc = settings->pc;
+  if (pc[0] < 1 || pc[1] < 1 || pc[2] < 1 || pc[4] <= 0 || pc[5] <= 0 || pc[6] <= 0) return WV_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return WV_ENODEV;
+  wv_solver* s = new wv_solver();
+  s->settings = *settings;
+  Ctx& c = s->ctx;
+  memset(&c, 0, sizeof c);
+  c.nx = (int)pc[0]; c.ny = (int)pc[1]; c.nz = (int)pc[2];
+  c.N = (int64_t)c.nx * c.ny * c.nz;
+  c.NU = (int64_t)(c.nx + 1) * c.ny * c.nz; c.NV = (int64_t)c.nx * (c.ny + 1) * c.nz; c.NW = (int64_t)c.nx * c.ny * (c.nz + 1);
+  c.NF = c.NU + c.NV + c.NW; c.stride = 2 * c.N + c.NF;
+  for (int a = 0; a < 3; a++) { c.h[a] = pc[4 + a]; c.o[a] = pc[8 + a]; }
+  c.dt_max = pc[7]; c.eta = pc[11]; c.perx = pc[12]; c.slope = pc[13]; c.corr = pc[14];
+  c.shape = pc[16]; c.gmult = pc[17]; c.max_face_cfl = pc[18]; c.hist_cap = pc[19];
+  for (int k = 0; k < 4; k++) { c.wave[k] = pc[20 + k]; c.forc[k] = pc[28 + k]; }
+  c.tol = pc[24]; c.linear_air = pc[25]; c.stokes2 = pc[26]; c.flags = (int)pc[27];
+  const float* o = settings->opts;
+  c.right_abs = o[0]; c.pert_amp = o[6]; c.seed = (uint32_t)o[7];
+  s->smoothing = (int)o[1] > 0 ? (int)o[1] : 3;
+  s->iterations = (int)o[5] > 0 ? (int)o[5] : 8;
+  s->history_capacity = (int64_t)pc[19];
+  if (!plan_levels(pc, (int)o[3], s->plan)) { delete s; return WV_EPLAN; }
+  int rc;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { delete s; return WV_EHIP; }
+#define ALLOC(ptr, n) if ((rc = dalloc(s, &(ptr), (size_t)(n))) != 0) { release(s); delete s; return rc; }
+  ALLOC(c.bank[0], c.stride); ALLOC(c.bank[1], c.stride);
+  ALLOC(c.work, 14 * c.N);
+  ALLOC(c.geo, c.N);
+  ALLOC(c.c, WV_NCTRL);
+  ALLOC(c.plane, c.N);
+  ALLOC(c.motion, 4 * c.N + 12 * c.NF);
+  ALLOC(c.history, s->history_capacity * 4 + 4);
+  ALLOC(c.chist, s->history_capacity * 5 + 5);
+  ALLOC(c.probe, 8);
+  ALLOC(c.img_planes, c.N);
+  ALLOC(c.img_kinds, c.N);
+  ALLOC(c.partial, (c.stride + WV_RED - 1) / WV_RED + 1);
+  ALLOC(c.counters, 64);
+  ALLOC(s->quota_dev, 4);
+  c.quota = s->quota_dev;
+  MGDev& m = s->mg;
+  memset(&m, 0, sizeof m);
+  m.L = s->plan.L; m.perx = c.perx > .5f;
+  for (int l = 0; l < m.L; l++) m.lv[l] = s->plan.lv[l];
+  int* offs; signed char* slots;
+  ALLOC(offs, s->plan.offsets.size());
+  ALLOC(slots, s->plan.slots.size());
+  ALLOC(m.vec, 8 * s->plan.node_count);
+  ALLOC(m.rows, s->plan.row_count);
+  int last = s->plan.lv[m.L - 1].count;
+  ALLOC(m.linv, (int64_t)last * last);
+  ALLOC(m.snapshot, 9 * c.N);
+  ALLOC(m.previous, 9 * c.N);
+  ALLOC(m.state, 64 + 32 * s->history_capacity);
+#undef ALLOC
+  m.offsets = offs; m.slots = slots;
+  m.lag = (int)o[2]; m.mode = (int)o[4]; m.capacity = (int)s->history_capacity;
+  if (hipMemcpy(offs, s->plan.offsets.data(), s->plan.offsets.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(slots, s->plan.slots.data(), s->plan.slots.size(), hipMemcpyHostToDevice) != hipSuccess) { release(s); delete s; return WV_EHIP; }
+  if (hipHostMalloc((void**)&s->quota_host, 64) != hipSuccess || hipHostMalloc((void**)&s->ctrl_host, WV_NCTRL * sizeof(float)) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ctrl_event, hipEventDisableTiming) != hipSuccess) { release(s); delete s; return WV_EHIP; }
+  memset(s->ctrl_host, 0, WV_NCTRL * sizeof(float));
+  *out = s;
+  return WV_OK;
+}
+
+extern "C" int wv_destroy(wv_solver* s) {
+  if (!s) return WV_EINVAL;
+  release(s);
+  delete s;
+  return WV_OK;
+}
+
+extern "C" const char* wv_last_error(wv_solver* s) { return s ? s->error.c_str() : "null solver"; }
+
+// Upload the quota (stream-ordered, outside any graph) when it changed.
+static int sync_quota(wv_solver* s) {
+  float q = (float)s->quota.load();
+  if (q == s->last_uploaded_quota) return 0;
+  HIPCHECK(s, hipStreamSynchronize(s->stream));  // the pinned slot may still be in flight
+  *s->quota_host = q;
+  HIPCHECK(s, hipMemcpyAsync(s->quota_dev, s->quota_host, sizeof(float), hipMemcpyHostToDevice, s->stream));
+  s->last_uploaded_quota = q;
+  return 0;
+}
+
+// water/pressure_gpu.gd:69-81: one op; the MG hooks are explicit in the batch.
+static int dispatch(wv_solver* s, int op) {
+  const Ctx& c = s->ctx;
+  if (op == 3) {
+    wv_launch_op(c, 3, s->stream);
+    wv_launch_pcg_init_partials(c, s->stream);
+    return 0;
+  }
+  if (op == 8) {
+    const MGLevel& L0 = s->mg.lv[0];
+    float* v = s->mg.vec + L0.vec_off;
+    wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, s->stream);
+    return 0;
+  }
+  int rc = wv_launch_op(c, op, s->stream);
+  if (rc != 0) return fail(s, WV_EINVAL, "unknown op " + std::to_string(op));
+  return 0;
+}
+
+// water/coupled_gpu.gd:38-56 + water/coastal_gpu.gd:28-33 with the MG hooks of
+// water/pressure_gpu.gd:70-74.
+static int record_batch(wv_solver* s) {
+  auto d = [&](int op) { return dispatch(s, op); };
+  d(43); d(1);
+  d(20); d(21); d(45); d(46); d(22);
+  for (int sweep = 0; sweep < 3; sweep++) { d(23); d(24); d(55); d(56); d(25); d(26); }
+  d(27); d(28);
+  for (int op : {80, 81, 82, 83, 84, 87, 88}) d(op);
+  d(17); d(18); d(16);
+  d(2); d(47); d(63); d(64);
+  wv_mg_build(s->ctx, s->mg, s->stream);
+  d(3);
+  wv_mg_cycle(s->ctx, s->mg, 0, s->smoothing, true, s->stream);  // + op 4
+  for (int it = 0; it < s->iterations; it++) {
+    d(5); d(6); d(7); d(8);
+    wv_mg_cycle(s->ctx, s->mg, 1, s->smoothing, false, s->stream);  // + op 9
+    d(10);
+  }
+  d(11); d(65); d(47);
+  for (int op : {30, 31, 32, 33, 34, 47, 48}) d(op);
+  d(35); d(36);
+  d(62); d(66); d(12); d(13); d(44); d(19);
+  d(85); d(89);
+  return 0;
+}
+
+static int ensure_graph(wv_solver* s) {
+  if (s->batch_exec) return 0;
+  HIPCHECK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  record_batch(s);
+  hipGraph_t g = nullptr;
+  HIPCHECK(s, hipStreamEndCapture(s->stream, &g));
+  s->batch_graph = g;
+  HIPCHECK(s, hipGraphInstantiate(&s->batch_exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int wv_reset(wv_solver* s) {
+  if (!s) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  s->quota.store(0.0);
+  int rc = sync_quota(s); if (rc) return rc;
+  // the reference allocates cleared buffers (water/pressure_gpu.gd:99); a reset
+  // here clears the same state in place
+  HIPCHECK(s, hipMemsetAsync(s->ctx.work, 0, 14 * s->ctx.N * sizeof(float), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->ctx.motion, 0, (4 * s->ctx.N + 12 * s->ctx.NF) * sizeof(float), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->mg.vec, 0, 8 * s->plan.node_count * sizeof(float), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->mg.snapshot, 0, 9 * s->ctx.N * sizeof(float), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->mg.previous, 0, 9 * s->ctx.N * sizeof(float), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->mg.state, 0, (64 + 32 * s->history_capacity) * sizeof(unsigned), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->ctx.history, 0, (s->history_capacity * 4 + 4) * sizeof(float4), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->ctx.chist, 0, (s->history_capacity * 5 + 5) * sizeof(float4), s->stream));
+  for (int op : {0, 40}) dispatch(s, op);
+  if (s->ctx.pert_amp != 0) dispatch(s, 90);
+  for (int op : {17, 18, 16, 19, 41, 42, 49, 85, 86}) dispatch(s, op);
+  HIPCHECK(s, hipGetLastError());
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  return WV_OK;
+}
+
+extern "C" int wv_request_steps(wv_solver* s, int64_t steps) {
+  if (!s || steps < 0) return WV_EINVAL;
+  s->quota.store((double)steps);
+  return WV_OK;
+}
+extern "C" int wv_request_time(wv_solver* s, double seconds) {
+  if (!s || !std::isfinite(seconds) || seconds < 0) return WV_EINVAL;
+  s->quota.store(-seconds);
+  return WV_OK;
+}
+
+extern "C" int wv_advance(wv_solver* s, int batches) {
+  if (!s || batches < 0) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  int rc = sync_quota(s); if (rc) return rc;
+  rc = ensure_graph(s); if (rc) return rc;
+  for (int b = 0; b < batches; b++) HIPCHECK(s, hipGraphLaunch(s->batch_exec, s->stream));
+  HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, s->ctx.c, WV_NCTRL * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHECK(s, hipEventRecord(s->ctrl_event, s->stream));
+  s->readback_pending = true;
+  return WV_OK;
+}
+
+extern "C" int wv_poll_control(wv_solver* s, float* out96, int* fresh, int wait) {
+  if (!s || !out96) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  if (fresh) *fresh = 0;
+  if (wait) HIPCHECK(s, hipStreamSynchronize(s->stream));
+  if (s->readback_pending) {
+    hipError_t e = hipEventQuery(s->ctrl_event);
+    if (e == hipSuccess) {
+      memcpy(s->last_control, s->ctrl_host, sizeof s->last_control);
+      s->readback_pending = false;
+      if (fresh) *fresh = 1;
+    } else if (e != hipErrorNotReady) return fail(s, WV_EHIP, std::string("control readback: ") + hipGetErrorString(e));
+  }
+  memcpy(out96, s->last_control, sizeof s->last_control);
+  return WV_OK;
+}
+
+static void* field_ptr(wv_solver* s, int field, int64_t* bytes) {
+  Ctx& c = s->ctx; MGDev& m = s->mg;
+  switch (field) {
+  case WV_FIELD_BANK_A: *bytes = c.stride * 4; return c.bank[0];
+  case WV_FIELD_BANK_B: *bytes = c.stride * 4; return c.bank[1];
+  case WV_FIELD_WORK: *bytes = 14 * c.N * 4; return c.work;
+  case WV_FIELD_GEO: *bytes = 2 * c.N * 4; return c.geo;
+  case WV_FIELD_CONTROL: *bytes = WV_NCTRL * 4; return c.c;
+  case WV_FIELD_PLANE: *bytes = 4 * c.N * 4; return c.plane;
+  case WV_FIELD_MOTION: *bytes = (4 * c.N + 12 * c.NF) * 4; return c.motion;
+  case WV_FIELD_IMG_PLANES: *bytes = 4 * c.N * 4; return c.img_planes;
+  case WV_FIELD_IMG_KINDS: *bytes = c.N * 4; return c.img_kinds;
+  case WV_FIELD_HISTORY: *bytes = s->history_capacity * 4 * 16; return c.history;
+  case WV_FIELD_COASTAL_HISTORY: *bytes = s->history_capacity * 5 * 16; return c.chist;
+  case WV_FIELD_PROBE: *bytes = 8 * 16; return c.probe;
+  case WV_FIELD_PARTIAL: *bytes = ((c.N + 127) / 128) * 16; return c.partial;
+  case WV_FIELD_MG_VECTORS: *bytes = 8 * s->plan.node_count * 4; return m.vec;
+  case WV_FIELD_SNAPSHOT: *bytes = 9 * c.N * 4; return m.snapshot;
+  case WV_FIELD_AUDIT: *bytes = (64 + 32 * s->history_capacity) * 4; return m.state;
+  case WV_FIELD_MG_ROWS: *bytes = s->plan.row_count * 4; return m.rows;
+  case WV_FIELD_MG_LINV: { int64_t n = s->plan.lv[s->plan.L - 1].count; *bytes = n * n * 4; return m.linv; }
+  }
+  *bytes = -1;
+  return nullptr;
+}
+
+extern "C" int64_t wv_field_bytes(wv_solver* s, int field) {
+  if (!s) return WV_EINVAL;
+  int64_t b; field_ptr(s, field, &b); return b;
+}
+extern "C" int wv_read_field(wv_solver* s, int field, void* dst, int64_t bytes) {
+  if (!s || !dst) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  int64_t n; void* p = field_ptr(s, field, &n);
+  if (!p || bytes < 0 || bytes > n) return fail(s, WV_EINVAL, "bad field or size");
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  HIPCHECK(s, hipMemcpy(dst, p, (size_t)bytes, hipMemcpyDeviceToHost));
+  return WV_OK;
+}
+extern "C" int wv_write_field(wv_solver* s, int field, const void* src, int64_t bytes) {
+  if (!s || !src) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  int64_t n; void* p = field_ptr(s, field, &n);
+  if (!p || bytes < 0 || bytes > n) return fail(s, WV_EINVAL, "bad field or size");
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  HIPCHECK(s, hipMemcpy(p, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return WV_OK;
+}
+
+extern "C" int wv_op(wv_solver* s, int op) {
+  if (!s) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  int rc = sync_quota(s); if (rc) return rc;
+  rc = dispatch(s, op); if (rc) return rc;
+  HIPCHECK(s, hipGetLastError());
+  return WV_OK;
+}
+
+extern "C" int wv_mg_record(wv_solver* s, int build, int initial) {
+  if (!s) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  if (build) wv_mg_build(s->ctx, s->mg, s->stream);
+  else wv_mg_cycle(s->ctx, s->mg, initial ? 0 : 1, s->smoothing, true, s->stream);
+  HIPCHECK(s, hipGetLastError());
+  return WV_OK;
+}
+
+extern "C" int wv_synchronize(wv_solver* s) {
+  if (!s) return WV_EINVAL;
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  return WV_OK;
+}
+
+extern "C" int wv_levels(wv_solver* s, int* meta16, int max_levels) {
+  if (!s) return WV_EINVAL;
+  int L = s->plan.L;
+  if (meta16) for (int l = 0; l < L && l < max_levels; l++) memcpy(meta16 + 16 * l, &s->plan.meta[16 * l], 64);
+  return L;
+}
+
+void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st);
+extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us) {
+  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 2) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  hipEvent_t a, b;
+  HIPCHECK(s, hipEventCreate(&a)); HIPCHECK(s, hipEventCreate(&b));
+  wv_launch_timed(s->ctx, s->mg, kernel, s->stream);  // warm
+  HIPCHECK(s, hipEventRecord(a, s->stream));
+  for (int r = 0; r < reps; r++) wv_launch_timed(s->ctx, s->mg, kernel, s->stream);
+  HIPCHECK(s, hipEventRecord(b, s->stream));
+  HIPCHECK(s, hipEventSynchronize(b));
+  float ms = 0; HIPCHECK(s, hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a); hipEventDestroy(b);
+  *avg_us = ms * 1000.0f / reps;
+  return WV_OK;
+}

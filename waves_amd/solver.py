@@ -1,0 +1,227 @@
+"""Host-side mirror of the reference solver runtime over the HIP C-ABI.
+
+The reference's host is GDScript: water/pressure_gpu.gd (lifecycle, request /
+request_time quota, telemetry decode), water/coupled_gpu.gd and
+water/coastal_gpu.gd (extra telemetry), water/work_scheduler.gd (frame-driven
+submission with one asynchronous control readback).  This module keeps those
+names and meanings on top of libwaves_hip.so (include/waves.h) through ctypes.
+There is no fallback: without the built library every entry raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Dict, Optional
+
+import numpy as np
+
+from .settings import Settings
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwaves_hip.so")
+
+FIELD_BANK_A, FIELD_BANK_B, FIELD_WORK, FIELD_GEO, FIELD_CONTROL = 0, 1, 2, 3, 4
+FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
+FIELD_HISTORY, FIELD_COASTAL_HISTORY, FIELD_PROBE, FIELD_PARTIAL = 9, 10, 11, 12
+FIELD_MG_VECTORS, FIELD_SNAPSHOT, FIELD_AUDIT, FIELD_MG_ROWS, FIELD_MG_LINV = 13, 14, 15, 16, 17
+
+#: every entry point declared in include/waves.h
+EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_request_time",
+           "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
+           "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
+           "wv_levels", "wv_time_kernel")
+
+
+class WavesError(RuntimeError):
+    pass
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [("pc", ctypes.c_float * 32), ("opts", ctypes.c_float * 8)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def library() -> ctypes.CDLL:
+    """Load libwaves_hip.so (raises if it has not been built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise WavesError(f"{LIB_PATH} is missing: run `python -m waves_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.wv_create.argtypes = [ctypes.POINTER(_Settings), ctypes.POINTER(vp)]
+        L.wv_destroy.argtypes = [vp]
+        L.wv_reset.argtypes = [vp]
+        L.wv_request_steps.argtypes = [vp, i64]
+        L.wv_request_time.argtypes = [vp, ctypes.c_double]
+        L.wv_advance.argtypes = [vp, i32]
+        L.wv_poll_control.argtypes = [vp, fp, ctypes.POINTER(i32), i32]
+        L.wv_field_bytes.restype = i64
+        L.wv_field_bytes.argtypes = [vp, i32]
+        L.wv_read_field.argtypes = [vp, i32, vp, i64]
+        L.wv_write_field.argtypes = [vp, i32, vp, i64]
+        L.wv_op.argtypes = [vp, i32]
+        L.wv_mg_record.argtypes = [vp, i32, i32]
+        L.wv_synchronize.argtypes = [vp]
+        L.wv_last_error.restype = ctypes.c_char_p
+        L.wv_last_error.argtypes = [vp]
+        L.wv_levels.argtypes = [vp, ctypes.POINTER(i32), i32]
+        L.wv_time_kernel.argtypes = [vp, i32, i32, fp]
+        _lib = L
+        return L
+
+
+class Solver:
+    """The reference's coastal solver object (water/presentation/coastal_view.gd
+    -> coastal_gpu.gd -> coupled_gpu.gd -> pressure_gpu.gd) on the HIP library."""
+
+    def __init__(self, settings: Settings):
+        self.settings = settings
+        self.config = settings
+        self.closed = False
+        self.ready = False
+        self.last_stats: Dict = {}
+        self._h = ctypes.c_void_p()
+        s = _Settings()
+        s.pc[:] = [float(v) for v in settings.push_constants()]
+        s.opts[:] = [float(v) for v in settings.options()]
+        self._lib = library()
+        rc = self._lib.wv_create(ctypes.byref(s), ctypes.byref(self._h))
+        if rc != 0:
+            raise WavesError(f"wv_create failed ({rc})")
+
+    # -- lifecycle: water/pressure_gpu.gd:45-59 ------------------------------
+    def start(self):
+        """Run the initial command list (water/coupled_gpu.gd:34-37)."""
+        self._check(self._lib.wv_reset(self._h))
+        self.ready = True
+
+    def close(self):
+        if self.closed:
+            return
+        self.closed = True
+        if self._h:
+            self._lib.wv_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self._lib.wv_last_error(self._h) if self._h else b""
+            raise WavesError(f"waves_hip error {rc}: {msg.decode(errors='replace') if msg else ''}")
+
+    # -- quota: water/pressure_gpu.gd:48-54 ----------------------------------
+    def request(self, steps: int):
+        self._check(self._lib.wv_request_steps(self._h, int(steps)))
+
+    def request_time(self, seconds: float):
+        assert np.isfinite(seconds) and seconds >= 0
+        self._check(self._lib.wv_request_time(self._h, float(seconds)))
+
+    # -- scheduling: water/work_scheduler.gd:41-87 ---------------------------
+    def frame(self, batches: int = 1):
+        """Enqueue whole batches and one control readback (non-blocking)."""
+        self._check(self._lib.wv_advance(self._h, int(batches)))
+
+    def poll(self, wait: bool = False) -> Optional[np.ndarray]:
+        """Newest control readback; returns None if nothing new arrived."""
+        out = np.zeros(96, np.float32)
+        fresh = ctypes.c_int(0)
+        self._check(self._lib.wv_poll_control(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(fresh), int(wait)))
+        if fresh.value or wait:
+            self.last_stats = self._stats(out)
+            return out
+        return None
+
+    def synchronize(self):
+        self._check(self._lib.wv_synchronize(self._h))
+
+    # -- telemetry: water/pressure_gpu.gd:133-137, coupled_gpu.gd:58-65,
+    #    coastal_gpu.gd:34-37 -------------------------------------------------
+    @staticmethod
+    def _stats(v: np.ndarray) -> Dict:
+        reported = v.copy()
+        recoverable = (v[7] == 20 or 10 <= v[7] <= 12) and v[29] < 2
+        if recoverable:
+            reported[7] = 0
+        r = reported
+        p = {"pressure_backend": "multigrid", "accepted_index": int(r[0]), "accepted_time": float(r[1]),
+             "accepted_steps": int(r[2]), "rejections": int(r[3]), "pending": bool(r[4] > 0.5),
+             "converged": bool(r[6] > 0.5), "failure": int(r[7]), "iterations": int(r[8]),
+             "relative_residual": float(r[13]), "max_dt_divergence": float(r[14]),
+             "max_face_speed": float(r[15]), "hydrostatic_error_pa": float(r[16]),
+             "volume_m3": float(r[17]), "total_iterations": int(r[18])}
+        p.update({"rhs_integrated_divergence_l2": float(v[62]), "residual_integrated_divergence_l2": float(v[63]),
+                  "true_relative_residual": float(v[60]), "pressure_defect_corrections": int(v[58]),
+                  "boundary_outflow_m3": float(v[40]), "tentative_outflow_m3": float(v[39]),
+                  "raw_compression_volume_m3": float(v[38]), "compression_volume_m3": float(v[35]),
+                  "compression_coverage_errors": int(v[36]), "step_start_volume_m3": float(v[37]),
+                  "trial_count": int(v[32]), "trial_failure": int(v[7]), "retry_count": int(v[29]),
+                  "last_rejection_code": int(v[33]), "last_rejected_dt": float(v[34]), "dt": float(v[20]),
+                  "signed_roundoff_m3": float(v[24]), "absolute_roundoff_m3": float(v[25]),
+                  "max_CFL": float(v[27]), "mass_defect_m3": float(v[28]), "initial_volume_m3": float(v[26]),
+                  "coupled": True})
+        p.update({"forcing_min_F": float(v[88]), "forcing_max_F": float(v[89]), "forcing_volume_m3": float(v[72]),
+                  "ocean_inflow_m3": float(v[73]), "ocean_outflow_m3": float(v[74]),
+                  "forcing_momentum_kg_mps": [float(x) for x in v[75:78]],
+                  "staged_forcing_volume_m3": float(v[64]), "staged_ocean_inflow_m3": float(v[65]),
+                  "staged_ocean_outflow_m3": float(v[66]), "gauges_m": [float(x) for x in v[80:88]]})
+        return p
+
+    # -- parity / export I/O -------------------------------------------------
+    def field_bytes(self, field: int) -> int:
+        return int(self._lib.wv_field_bytes(self._h, field))
+
+    def read(self, field: int, dtype=np.float32) -> np.ndarray:
+        n = self.field_bytes(field)
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype)
+        self._check(self._lib.wv_read_field(self._h, field, out.ctypes.data, n))
+        return out
+
+    def write(self, field: int, values: np.ndarray):
+        v = np.ascontiguousarray(values)
+        self._check(self._lib.wv_write_field(self._h, field, v.ctypes.data, v.nbytes))
+
+    def op(self, op: int):
+        self._check(self._lib.wv_op(self._h, int(op)))
+
+    def mg_record(self, build: bool, initial: bool = False):
+        self._check(self._lib.wv_mg_record(self._h, int(build), int(initial)))
+
+    @property
+    def control(self) -> np.ndarray:
+        return self.read(FIELD_CONTROL)
+
+    def levels(self):
+        meta = (ctypes.c_int * (16 * 12))()
+        L = self._lib.wv_levels(self._h, meta, 12)
+        return [list(meta[16 * l:16 * l + 16]) for l in range(L)]
+
+    def time_kernel(self, kernel: int, reps: int = 50) -> float:
+        out = ctypes.c_float(0)
+        self._check(self._lib.wv_time_kernel(self._h, kernel, reps, ctypes.byref(out)))
+        return float(out.value)
+
+    def step(self, max_batches: int = 64) -> int:
+        """Advance until one more step is accepted (or the run stops); one
+        control readback per batch. Returns the batches used."""
+        target = int(self.control[2]) + 1
+        self.request(target)
+        for b in range(1, max_batches + 1):
+            self.frame(1)
+            c = self.poll(wait=True)
+            if c[2] >= target or c[7] > 0.5:
+                return b
+        return max_batches
