@@ -35,6 +35,25 @@ def test_initial_commands_match(gpu):
     np.testing.assert_allclose(c_g[80:88], c_o[80:88], rtol=0, atol=0)
 
 
+def mg_report(o, g):
+    """After a build: active masks bit-exact, l1 scales and coarse rows within
+    FP32 reordering of the Galerkin sums."""
+    vo, vg = o.get(P.O.FIELD_MG_VECTORS), g.read(P.S.FIELD_MG_VECTORS)
+    ro = o.get(P.O.FIELD_MG_ROWS)[0::2]
+    rg = g.read(P.S.FIELD_MG_ROWS)
+    for l, m in enumerate(o.levels()):
+        off, cnt = m[3], m[8]
+        act_o, act_g = vo[8 * off + 3 * cnt: 8 * off + 4 * cnt], vg[8 * off + 3 * cnt: 8 * off + 4 * cnt]
+        assert np.array_equal(act_o, act_g), ("active mask", l)
+        d_o, d_g = vo[8 * off + 4 * cnt: 8 * off + 5 * cnt], vg[8 * off + 4 * cnt: 8 * off + 5 * cnt]
+        rel = float(np.abs(d_o - d_g).max() / max(np.abs(d_o).max(), 1e-30))
+        say("  level", l, "count", cnt, "l1 rel diff", rel)
+        assert rel < 1e-5 if l > 0 else rel == 0
+    rel = float(np.abs(ro - rg).max() / max(np.abs(ro).max(), 1e-30))
+    say("  coarse rows rel diff", rel)
+    assert rel < 1e-5
+
+
 def _batch_report(s, quota=1.0, groups=None, check=True):
     o, g = P.make_pair(s)
     P.start_pair(o, g)
@@ -50,6 +69,9 @@ def _batch_report(s, quota=1.0, groups=None, check=True):
         rep = P.compare(o, g, P.STATE_FIELDS)
         if "build" in grp:
             rep.update(P.compare(o, g, (P.O.FIELD_SNAPSHOT, P.O.FIELD_AUDIT)))
+            mg_report(o, g)
+        if any(x in ("cycle0", "cycle1") for x in grp):
+            rep.update(P.compare(o, g, (P.O.FIELD_MG_VECTORS,)))
         bad = {k: v for k, v in rep.items() if v[0]}
         say("group", grp, "c[4..8]", o.control[4:9], bad if bad else "exact")
         for k, v in bad.items():

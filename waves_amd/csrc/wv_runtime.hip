@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -131,6 +132,7 @@ struct wv_solver {
   hipGraphExec_t batch_exec = nullptr;
   hipGraph_t batch_graph = nullptr;
   int smoothing = 3, iterations = 8;
+  bool use_graph = true;  // WAVES_HIP_GRAPH=0: launch each batch's kernels directly (tracing tools)
   int64_t history_capacity = 0;
   std::string error;
   std::mutex lock;
@@ -195,6 +197,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   s->smoothing = (int)o[1] > 0 ? (int)o[1] : 3;
   s->iterations = (int)o[5] > 0 ? (int)o[5] : 8;
   s->history_capacity = (int64_t)pc[19];
+  { const char* e = getenv("WAVES_HIP_GRAPH"); if (e && e[0] == '0') s->use_graph = false; }
   if (!plan_levels(pc, (int)o[3], s->plan)) { delete s; return WV_EPLAN; }
   int rc;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { delete s; return WV_EHIP; }
@@ -355,8 +358,13 @@ extern "C" int wv_advance(wv_solver* s, int batches) {
   if (!s || batches < 0) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
   int rc = sync_quota(s); if (rc) return rc;
-  rc = ensure_graph(s); if (rc) return rc;
-  for (int b = 0; b < batches; b++) HIPCHECK(s, hipGraphLaunch(s->batch_exec, s->stream));
+  if (s->use_graph) {
+    rc = ensure_graph(s); if (rc) return rc;
+    for (int b = 0; b < batches; b++) HIPCHECK(s, hipGraphLaunch(s->batch_exec, s->stream));
+  } else {
+    for (int b = 0; b < batches; b++) record_batch(s);
+    HIPCHECK(s, hipGetLastError());
+  }
   HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, s->ctx.c, WV_NCTRL * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   HIPCHECK(s, hipEventRecord(s->ctrl_event, s->stream));
   s->readback_pending = true;
