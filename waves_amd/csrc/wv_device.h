@@ -42,17 +42,20 @@ struct Ctx {
   float4* img_planes;
   float* img_kinds;
   float4* partial;
-  unsigned* counters;  // last-block tickets, one per fused reduction kernel
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
 };
 
 struct I3 { int x, y, z; };
 __device__ __forceinline__ I3 mk(int x, int y, int z) { I3 r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ int& comp(I3& q, int a) { return a == 0 ? q.x : (a == 1 ? q.y : q.z); }
+// Component access by a run-time axis through selects: a reference to a
+// struct member would take its address and push the struct to scratch memory.
+__device__ __forceinline__ void addc(I3& q, int a, int v) { q.x += a == 0 ? v : 0; q.y += a == 1 ? v : 0; q.z += a == 2 ? v : 0; }
+__device__ __forceinline__ void setc(I3& q, int a, int v) { q.x = a == 0 ? v : q.x; q.y = a == 1 ? v : q.y; q.z = a == 2 ? v : q.z; }
 __device__ __forceinline__ int compc(const I3& q, int a) { return a == 0 ? q.x : (a == 1 ? q.y : q.z); }
 struct F3 { float x, y, z; };
 __device__ __forceinline__ F3 mkf(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ float& comp(F3& q, int a) { return a == 0 ? q.x : (a == 1 ? q.y : q.z); }
+__device__ __forceinline__ void addc(F3& q, int a, float v) { if (a == 0) q.x += v; else if (a == 1) q.y += v; else q.z += v; }
+__device__ __forceinline__ void setc(F3& q, int a, float v) { q.x = a == 0 ? v : q.x; q.y = a == 1 ? v : q.y; q.z = a == 2 ? v : q.z; }
 __device__ __forceinline__ float compc(const F3& q, int a) { return a == 0 ? q.x : (a == 1 ? q.y : q.z); }
 __device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float len3(F3 a) { return sqrtf(dot3(a, a)); }
@@ -137,13 +140,13 @@ __device__ __forceinline__ int64_t face_index(const Ctx& s, I3 q, int axis) {
   I3 n = dims(s);
   if (axis == 2 && q.z == n.z) q.z = 0;
   if (axis == 0 && s.perx > .5f && q.x == n.x) q.x = 0;
-  comp(n, axis)++;
+  addc(n, axis, 1);
   return (axis == 0 ? U_(s) : (axis == 1 ? V_(s) : W_(s))) + index3(q, n);
 }
 __device__ __forceinline__ float divergence(const Ctx& s, I3 q) {
   int cb = candidate(s); float d = 0;
   for (int axis = 0; axis < 3; axis++) {
-    I3 r = q; comp(r, axis)++;
+    I3 r = q; addc(r, axis, 1);
     d += (getv(s, cb, face_index(s, r, axis)) - getv(s, cb, face_index(s, q, axis))) / s.h[axis];
   }
   return d;
@@ -171,7 +174,7 @@ __device__ __forceinline__ int64_t face_local(const Ctx& s, I3 q, int axis) { re
 __device__ __forceinline__ void unpack_face(const Ctx& s, int64_t f, I3& q, int& axis) {
   axis = f < s.NU ? 0 : (f < s.NU + s.NV ? 1 : 2);
   int64_t base = axis == 0 ? 0 : (axis == 1 ? s.NU : s.NU + s.NV);
-  I3 n = dims(s); comp(n, axis)++; q = coord(f - base, n);
+  I3 n = dims(s); addc(n, axis, 1); q = coord(f - base, n);
 }
 
 // ---- wavefront-shuffle reductions -------------------------------------------
@@ -218,19 +221,4 @@ __device__ __forceinline__ float4 global_reduce(const float4* partial, int64_t g
     v.x += p.x; v.y += p.y; v.z += p.z; v.w = MAXW ? gmax(v.w, p.w) : v.w + p.w;
   }
   return group_reduce<MAXW ? RED_PRESSURE : RED_SUM>(v);
-}
-// Last-block ticket: every 128-lane block publishes its partial, and the block
-// that takes the final ticket runs the global step.  Returns true in that block.
-__device__ __forceinline__ bool last_block(unsigned* ticket) {
-  __shared__ bool is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1);
-    if (is_last) *ticket = 0;  // reset for the next launch (kernel end orders it)
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
 }

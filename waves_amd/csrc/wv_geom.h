@@ -69,7 +69,7 @@ __device__ __forceinline__ float coastal_velocity_axis(const Ctx& s, F3 world, f
   if (axis == 2) return 0.0f;
   F3 v = coastal_velocity(s, world, time); return axis == 0 ? v.x : v.y;
 }
-__device__ __forceinline__ F3 coastal_face_position(const Ctx& s, I3 q, int axis) { F3 p = center(s, q); comp(p, axis) -= .5f * s.h[axis]; return p; }
+__device__ __forceinline__ F3 coastal_face_position(const Ctx& s, I3 q, int axis) { F3 p = center(s, q); addc(p, axis, -(.5f * s.h[axis])); return p; }
 __device__ __forceinline__ float4 coastal_plane(const Ctx& s, I3 q, float time) {
   float se, ss; coastal_surface(s, center(s, q).x, time, se, ss);
   F3 normal = mkf(-ss * s.h[0], s.h[1], 0);
@@ -113,8 +113,8 @@ __device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source,
     int b = (axis + 1) % 3, d = (axis + 2) % 3, weights = 0;
     float acc = 0;
     for (int j = -1; j <= 1; j++) for (int k = -1; k <= 1; k++) {
-      I3 middle = q; comp(middle, b) += j; comp(middle, d) += k; if (solid(s, middle)) continue;
-      I3 l = middle, r = middle; comp(l, axis)--; comp(r, axis)++;
+      I3 middle = q; addc(middle, b, j); addc(middle, d, k); if (solid(s, middle)) continue;
+      I3 l = middle, r = middle; addc(l, axis, -1); addc(r, axis, 1);
       bool left = !solid(s, l), right = !solid(s, r); if (!left && !right) continue;
       float difference;
       if (left && right) difference = hf_fraction(s, l, source, gbank) - hf_fraction(s, r, source, gbank);
@@ -124,13 +124,13 @@ __device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source,
       acc += (float)weight * difference / (32 * s.h[axis]);
     }
     if (weights > 0) acc *= 16.0f / (float)weights;
-    comp(g, axis) = acc;
+    setc(g, axis, acc);
   }
   return g;
 }
 __device__ __forceinline__ bool hf_valid(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && !solid(s, q); }
 // height_normal.glslinc:6-34 (numerics flag bit 0)
-__device__ __noinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int source, int gbank) {
+__device__ __forceinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int source, int gbank) {
   F3 scaled = mkf(fabsf(youngs.x) * s.h[0], fabsf(youngs.y) * s.h[1], fabsf(youngs.z) * s.h[2]);
   int axis = scaled.x > scaled.y ? 0 : 1; if (scaled.z > compc(scaled, axis)) axis = 2;
   if (compc(scaled, axis) < 1e-12f) return youngs;
@@ -138,11 +138,11 @@ __device__ __noinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int sour
   float heights[5];
   for (int column = 0; column < 5; column++) {
     I3 base = q;
-    if (column == 1) comp(base, b)--; if (column == 2) comp(base, b)++;
-    if (column == 3) comp(base, d)--; if (column == 4) comp(base, d)++;
+    if (column == 1) addc(base, b, -1); if (column == 2) addc(base, b, 1);
+    if (column == 3) addc(base, d, -1); if (column == 4) addc(base, d, 1);
     float sum = 0, prev = orientation > 0 ? 1 : 0;
     for (int t = -4; t <= 4; t++) {
-      I3 cell = base; comp(cell, axis) += t; if (!hf_valid(s, cell)) return youngs;
+      I3 cell = base; addc(cell, axis, t); if (!hf_valid(s, cell)) return youngs;
       float f = hf_fraction(s, cell, source, gbank);
       if (isnan(f) || isinf(f) || f < 0 || f > 1) return youngs;
       if (orientation * (f - prev) > 2e-6f) return youngs;
@@ -152,9 +152,9 @@ __device__ __noinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int sour
     }
     heights[column] = sum * s.h[axis];
   }
-  F3 r = mkf(0, 0, 0); comp(r, axis) = orientation;
-  comp(r, b) = -(heights[2] - heights[1]) / (2 * s.h[b]);
-  comp(r, d) = -(heights[4] - heights[3]) / (2 * s.h[d]);
+  F3 r = mkf(0, 0, 0); setc(r, axis, orientation);
+  setc(r, b, -(heights[2] - heights[1]) / (2 * s.h[b]));
+  setc(r, d, -(heights[4] - heights[3]) / (2 * s.h[d]));
   return r;
 }
 // pressure_geometry.glslinc:91-98 / coupled_motion.glslinc:50-56
@@ -175,7 +175,7 @@ __device__ __forceinline__ float segment_distance(F3 p, F3 a, F3 b) {
   return len3(mkf(p.x - (a.x + t * ab.x), p.y - (a.y + t * ab.y), p.z - (a.z + t * ab.z)));
 }
 // pressure_geometry.glslinc:105-120: distance to the PLIC polygon of a cell
-__device__ __noinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq) {
+__device__ __forceinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq) {
   F3 h = mkf(s.h[0], s.h[1], s.h[2]);
   F3 normal = mkf(eq.x / h.x, eq.y / h.y, eq.z / h.z);
   float norm2 = dot3(normal, normal), step = dot3(normal, point) - eq.w;
@@ -185,10 +185,10 @@ __device__ __noinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq
   F3 vert[12]; int count = 0;
   float e3[3] = {eq.x, eq.y, eq.z};
   for (int axis = 0; axis < 3; axis++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
-    F3 a = mkf(0, 0, 0); comp(a, (axis + 1) % 3) = (float)j; comp(a, (axis + 2) % 3) = (float)k;
+    F3 a = mkf(0, 0, 0); setc(a, (axis + 1) % 3, (float)j); setc(a, (axis + 2) % 3, (float)k);
     float den = e3[axis]; if (fabsf(den) < 1e-15f) continue;
     float t = (eq.w - dot3(mkf(eq.x, eq.y, eq.z), a)) / den;
-    if (t >= 0 && t <= 1) { comp(a, axis) = t; vert[count++] = mkf(a.x * h.x, a.y * h.y, a.z * h.z); }
+    if (t >= 0 && t <= 1) { setc(a, axis, t); vert[count++] = mkf(a.x * h.x, a.y * h.y, a.z * h.z); }
   }
   float best = 1e10f;
   for (int j = 0; j < count; j++) {
@@ -213,8 +213,8 @@ __device__ __forceinline__ int clip_polygon(F3* poly, int count, float4 eq) {
   return size;
 }
 // pressure_geometry.glslinc:133-156: represented phase transitions on a cell face
-__device__ __noinline__ float transition_distance(const Ctx& s, F3 point, I3 left, int axis, int gbank) {
-  I3 right = left; comp(right, axis)++;
+__device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 left, int axis, int gbank) {
+  I3 right = left; addc(right, axis, 1);
   I3 l = wrap_cell(s, left), r = wrap_cell(s, right);
   if (outside(s, l) || outside(s, r) || solid(s, l) || solid(s, r)) return 1e10f;
   float fl = geometry_fraction(s, l, gbank), fr = geometry_fraction(s, r, gbank);
@@ -226,12 +226,12 @@ __device__ __noinline__ float transition_distance(const Ctx& s, F3 point, I3 lef
   float best = 1e10f; F3 h = mkf(s.h[0], s.h[1], s.h[2]);
   for (int side = 0; side < 2; side++) {
     F3 poly[12]; int j = (axis + 1) % 3, k = (axis + 2) % 3;
-    for (int v = 0; v < 4; v++) { poly[v] = mkf(0, 0, 0); comp(poly[v], axis) = 1; comp(poly[v], j) = (v == 1 || v == 2) ? 1 : 0; comp(poly[v], k) = v >= 2 ? 1 : 0; }
+    for (int v = 0; v < 4; v++) { poly[v] = mkf(0, 0, 0); setc(poly[v], axis, 1); setc(poly[v], j, (v == 1 || v == 2) ? 1 : 0); setc(poly[v], k, v >= 2 ? 1 : 0); }
     float4 first = side == 0 ? a : b;
     float4 second = side == 0 ? make_float4(-b.x, -b.y, -b.z, -b.w) : make_float4(-a.x, -a.y, -a.z, -a.w);
     int count = clip_polygon(poly, 4, first); count = clip_polygon(poly, count, second);
     if (count == 0) continue;
-    F3 projected = point; comp(projected, axis) = compc(h, axis);
+    F3 projected = point; setc(projected, axis, compc(h, axis));
     F3 local = mkf(projected.x / h.x, projected.y / h.y, projected.z / h.z);
     if (local.x >= 0 && local.y >= 0 && local.z >= 0 && local.x <= 1 && local.y <= 1 && local.z <= 1 &&
         dot3(mkf(first.x, first.y, first.z), local) <= first.w && dot3(mkf(second.x, second.y, second.z), local) <= second.w)
@@ -252,7 +252,7 @@ __device__ __forceinline__ float geometry_distance(const Ctx& s, I3 q, int gbank
   }
   bool needs_seed = false;
   for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
-    I3 nb = q; comp(nb, axis) += sign; nb = wrap_cell(s, nb); if (solid(s, nb) || outside(s, nb)) continue;
+    I3 nb = q; addc(nb, axis, sign); nb = wrap_cell(s, nb); if (solid(s, nb) || outside(s, nb)) continue;
     float other = geometry_fraction(s, nb, gbank); bool inside = other >= 1;
     if (other > 0 && other < 1) { float4 e = s.plane[index3(nb, dims(s))]; inside = dot3(mkf(e.x, e.y, e.z), mkf(.5f, .5f, .5f)) < e.w; }
     if (inside != (f >= 1)) needs_seed = true;

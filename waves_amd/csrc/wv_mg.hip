@@ -50,7 +50,7 @@ __device__ __forceinline__ float mapply(const MGDev& m, int l, int field, int i)
     float here = m.vec[at(m, 0, field, i)];
 #pragma unroll
     for (int f6 = 0; f6 < 6; f6++) {
-      int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+      int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
       int j = mwrap(m, 0, nb);
       float other = mactive(m, 0, j) ? m.vec[at(m, 0, field, j)] : 0;
       sum += snap(m, 6 + f6, i) * (here - other);
@@ -93,13 +93,13 @@ __device__ __forceinline__ bool mg_enabled(const Ctx& s, int phase) {
 
 // ------------------------------------------------------------ V-cycle -------
 // pressure_multigrid.glslinc:191 (op 5): b = r, x1 = x2 = 0 on level 0
-__global__ void k_mg_load(Ctx s, MGDev m, int phase) {
+__global__ void __launch_bounds__(256) k_mg_load(Ctx s, MGDev m, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   m.vec[at(m, 0, 0, i)] = s.work[i]; m.vec[at(m, 0, 1, i)] = 0; m.vec[at(m, 0, 2, i)] = 0;
 }
 // pressure_multigrid.glslinc:192-196: 4th-kind Chebyshev sweep k, l1-Jacobi scaled
-__global__ void k_mg_cheb(Ctx s, MGDev m, int l, int source, int k, int phase) {
+__global__ void __launch_bounds__(256) k_mg_cheb(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
@@ -108,20 +108,20 @@ __global__ void k_mg_cheb(Ctx s, MGDev m, int l, int source, int k, int phase) {
 }
 // terminal smoother when coarsening stops above 64 nodes
 // (water/pressure_multigrid.gd:144-149): op 14 then op 6 sweeps
-__global__ void k_mg_jacobi(Ctx s, MGDev m, int l, int source, int first, int phase) {
+__global__ void __launch_bounds__(256) k_mg_jacobi(Ctx s, MGDev m, int l, int source, int first, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   if (first) { m.vec[at(m, l, 2, i)] = m.vec[at(m, l, 4, i)] * m.vec[at(m, l, 0, i)]; return; }
   m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + m.vec[at(m, l, 4, i)] * (m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i));
 }
 // op 7
-__global__ void k_mg_residual(Ctx s, MGDev m, int l, int field, int phase) {
+__global__ void __launch_bounds__(256) k_mg_residual(Ctx s, MGDev m, int l, int field, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   m.vec[at(m, l, 6, i)] = m.vec[at(m, l, 0, i)] - mapply(m, l, field, (int)i);
 }
 // op 8: exact P^T residual gather into level l from level l-1
-__global__ void k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
+__global__ void __launch_bounds__(256) k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   int parent = l - 1; const MGLevel& P = m.lv[parent];
@@ -160,7 +160,7 @@ __global__ void k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
   m.vec[at(m, l, 0, i)] = sum; m.vec[at(m, l, 1, i)] = 0; m.vec[at(m, l, 2, i)] = 0;
 }
 // op 9: x_l += P x_{l+1}
-__global__ void k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
+__global__ void __launch_bounds__(256) k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   float add = 0;
@@ -168,7 +168,7 @@ __global__ void k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
   m.vec[at(m, l, ff, i)] += add;
 }
 // pressure_mg_coarse.glslinc:50-53 (op 12): z = L^-T (L^-1 b), one group
-__global__ void k_mg_coarse_solve(Ctx s, MGDev m, int l, int phase) {
+__global__ void __launch_bounds__(256) k_mg_coarse_solve(Ctx s, MGDev m, int l, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ float rhs[64];
   int lid = threadIdx.x, N = m.lv[l].count; const float* Li = m.linv;
@@ -178,8 +178,9 @@ __global__ void k_mg_coarse_solve(Ctx s, MGDev m, int l, int phase) {
 }
 // pressure_multigrid.glslinc:230-234 (op 10) fused with the PCG finaliser:
 // phase 0 -> pressure_iterations.glslinc:14-19 (op 4), phase 1 -> :31 (op 9)
-__global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, int phase, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, int phase, int final) {
   if (!mg_enabled(s, phase)) return;
+  if (!final) {
   int64_t i = GID(); int64_t N = s.N; float4 value = make_float4(0, 0, 0, 0);
   if (i < N) {
     float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i];
@@ -188,8 +189,10 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, in
   }
   value = group_reduce<RED_PRESSURE>(value);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = value;
-  if (!last_block(ticket)) return;
-  float4 v = global_reduce<true>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 v = global_reduce<true>(s.partial, G);
   if (threadIdx.x != 0) return;
   float* c = s.c;
   if (phase == 0) {
@@ -207,15 +210,18 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, in
 }
 // op 3 when the V-cycle is predicated off (a failure code is set): the
 // reference's op 4 then reduces op 3's own partials.
-__global__ void __launch_bounds__(WV_RED) k_pcg_init_partials(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_pcg_init_partials(Ctx s, int final) {
   if (!pressure_setup(s)) return;
   if (mg_enabled(s, 0)) return;
+  if (!final) {
   int64_t i = GID(); int64_t N = s.N; float4 v = make_float4(0, 0, 0, 0);
   if (i < N) { float r = s.work[i], z = s.work[N + i], bv = s.work[5 * N + i]; v = make_float4(r * z, r * r, bv * bv, fabsf(r)); }
   v = group_reduce<RED_PRESSURE>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<true>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<true>(s.partial, G);
   if (threadIdx.x != 0) return;
   float* c = s.c;
   c[9] = g.x; c[10] = g.z; c[13] = sqrtf(g.y / gmax(g.z, 1e-30f));
@@ -230,7 +236,7 @@ __device__ __forceinline__ unsigned signature(const Ctx& s, int64_t i) {
   if (s.geo[i].y < .5f) return 0;
   I3 n = dims(s), q = coord(i, n); unsigned result = 0;
   for (int f6 = 0; f6 < 6; f6++) {
-    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
     nb.z = pmod(nb.z, n.z); if (s.perx > .5f) nb.x = pmod(nb.x, n.x);
     bool inside = nb.x >= 0 && nb.x < n.x && nb.y >= 0 && nb.y < n.y;
     bool w = inside && s.geo[index3(nb, n)].y > .5f;
@@ -240,12 +246,12 @@ __device__ __forceinline__ unsigned signature(const Ctx& s, int64_t i) {
   return result;
 }
 __device__ __forceinline__ bool audit_on(const Ctx& s) { return !(s.c[5] < .5f || s.c[7] > .5f); }
-__global__ void k_audit0(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_audit0(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   unsigned* st = m.state;
   for (int k = 1; k <= 7; k++) st[k] = 0; st[14] = st[0]; st[15] = 0; st[16] = 0; for (int k = 20; k < 32; k++) st[k] = 0;
 }
-__global__ void k_audit1(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_audit1(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   __shared__ unsigned cnt[9];
   if (threadIdx.x < 9) cnt[threadIdx.x] = 0;
@@ -276,7 +282,7 @@ __global__ void k_audit1(Ctx s, MGDev m) {
     if (cnt[7]) atomicAdd(&st[15], cnt[7]); if (cnt[8]) atomicAdd(&st[16], cnt[8]);
   }
 }
-__global__ void k_audit2(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_audit2(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   unsigned* st = m.state;
   if (st[15] > 0) { s.c[7] = 34; return; }
@@ -291,7 +297,7 @@ __global__ void k_audit2(Ctx s, MGDev m) {
     st[row + 9] = st[8]; st[row + 10] = st[9]; st[row + 11] = st[10]; st[row + 12] = st[16]; st[row + 13] = st[11];
   }
 }
-__global__ void k_audit3(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_audit3(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   int64_t i = GID(), n = s.N; if (i >= n) return;
   bool numeric = m.state[8] != 0;
@@ -300,7 +306,7 @@ __global__ void k_audit3(Ctx s, MGDev m) {
   m.previous[7 * n + i] = kind; m.previous[8 * n + i] = sig;
   if (numeric) { m.snapshot[7 * n + i] = kind; m.snapshot[8 * n + i] = sig; }
 }
-__global__ void k_audit4(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_audit4(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   unsigned* st = m.state;
   if (st[13] <= (unsigned)m.capacity) { int64_t row = 64 + 32 * ((int64_t)st[13] - 1); for (int l = 0; l < 12; l++) st[row + 16 + l] = st[20 + l]; }
@@ -312,7 +318,7 @@ __device__ __forceinline__ bool build_on(const Ctx& s, const MGDev& m, bool stru
   return structural ? m.state[9] != 0 : m.state[8] != 0;
 }
 // pressure_multigrid.glslinc:139-150 (op 1)
-__global__ void k_mg_classify(Ctx s, MGDev m, int l) {
+__global__ void __launch_bounds__(256) k_mg_classify(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, true)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   float value = 0;
@@ -327,19 +333,19 @@ __global__ void k_mg_classify(Ctx s, MGDev m, int l) {
   m.vec[at(m, l, 3, i)] = value;
 }
 // pressure_multigrid.glslinc:183-189 (op 4), level 0
-__global__ void k_mg_l1_fine(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_mg_l1_fine(Ctx s, MGDev m) {
   if (!build_on(s, m, false)) return;
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   I3 q = coord(i, mdims(m, 0)); float sum = snap(m, 12, i);
   for (int f6 = 0; f6 < 6; f6++) {
-    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1;
+    int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
     if (mactive(m, 0, mwrap(m, 0, nb))) sum += snap(m, 6 + f6, i);
   }
   if (!(sum > 0) || isnan(sum) || isinf(sum)) { s.c[7] = 32; return; }
   m.vec[at(m, 0, 4, i)] = 1 / sum;
 }
 // op 4 at a coarse level: l1 row sums of the (symmetric) Galerkin rows
-__global__ void k_mg_l1_coarse(Ctx s, MGDev m, int l) {
+__global__ void __launch_bounds__(256) k_mg_l1_coarse(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
   int64_t i = GID(); int N = m.lv[l].count; if (i >= N) return;
   const float* row = m.rows + m.lv[l].row_off + i; float sum = 0;
@@ -382,7 +388,7 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin(Ctx s, MGDev m, int
         int j; float aij;
         if (l == 0) {
           if (k == 0) { j = i; aij = snap(m, 12, i); }
-          else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; I3 nb = q; comp(nb, axis) += 2 * side - 1; j = mwrap(m, 0, nb); if (j < 0) continue; aij = -snap(m, 6 + f6, i); }
+          else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1); j = mwrap(m, 0, nb); if (j < 0) continue; aij = -snap(m, 6 + f6, i); }
         } else {
           I3 off = moffset(m, l, k); j = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z)); if (j < 0) continue;
           aij = m.rows[F.row_off + (int64_t)k * F.count + i];
@@ -408,7 +414,7 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin(Ctx s, MGDev m, int
 }
 // Keep the canonical (upper) entry of every coupling and mirror it, so the
 // coarse operator is exactly symmetric; non-canonical duplicate offsets stay 0.
-__global__ void k_mg_symmetrize(Ctx s, MGDev m, int l) {
+__global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
   int64_t I = GID(); const MGLevel& C = m.lv[l]; if (I >= C.count) return;
   I3 q = coord(I, mdims(m, l)); const signed char* slots = m.slots + 125 * l;
@@ -425,7 +431,7 @@ __global__ void k_mg_symmetrize(Ctx s, MGDev m, int l) {
   }
 }
 // pressure_mg_coarse.glslinc:35-49 (op 11): dense Cholesky in LDS, store L^-1
-__global__ void k_mg_cholesky(Ctx s, MGDev m, int l) {
+__global__ void __launch_bounds__(256) k_mg_cholesky(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
   __shared__ float dense[64 * 64], inv[64 * 64];
   int lid = threadIdx.x, N = m.lv[l].count; const signed char* slots = m.slots + 125 * l;
@@ -510,24 +516,26 @@ void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool lo
   if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
   CycleBuilder b{s, m, phase, smoothing, st};
   int final_field = b.cycle(0);
-  hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, s.counters + 16 + phase);
+  hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
+  hipLaunchKernelGGL(k_mg_out, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
 }
 
 // op 3's partials + op 4 when the V-cycle is predicated off
 void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_pcg_init_partials, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, s.counters + 20);
+  hipLaunchKernelGGL(k_pcg_init_partials, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_pcg_init_partials, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
 
 // Hot kernels launched on their own for live timing (the kernels' outputs are
 // V-cycle / PCG scratch that every use recomputes first); phase 3 = ungated.
-__global__ void k_mg_cheb_timed(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_mg_cheb_timed(Ctx s, MGDev m) {
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   int l = 0, source = 1, k = 2;
   float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
   float d = (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)] + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 2, i)] = m.vec[at(m, l, 1, i)] + d;
 }
-__global__ void k_mg_residual_timed(Ctx s, MGDev m) {
+__global__ void __launch_bounds__(256) k_mg_residual_timed(Ctx s, MGDev m) {
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   m.vec[at(m, 0, 6, i)] = m.vec[at(m, 0, 0, i)] - mapply(m, 0, 1, (int)i);
 }

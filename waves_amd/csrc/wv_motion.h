@@ -9,9 +9,9 @@ __device__ __forceinline__ F3 periodic_delta(const Ctx& s, I3 a, I3 b) {
   if (s.perx > .5f) d.x -= roundf(d.x / (float)s.nx) * (float)s.nx;
   return mkf(d.x * s.h[0], d.y * s.h[1], d.z * s.h[2]);
 }
-__device__ __forceinline__ bool wall_face(const Ctx& s, I3 q, int axis) { I3 l = q; comp(l, axis)--; return solid(s, q) || solid(s, l); }
+__device__ __forceinline__ bool wall_face(const Ctx& s, I3 q, int axis) { I3 l = q; addc(l, axis, -1); return solid(s, q) || solid(s, l); }
 __device__ __forceinline__ bool face_in_bounds(const Ctx& s, I3 q, int axis) {
-  I3 n = dims(s); comp(n, axis)++;
+  I3 n = dims(s); addc(n, axis, 1);
   return q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < n.x && q.y < n.y && q.z < n.z;
 }
 __device__ __forceinline__ I3 wrap_face(const Ctx& s, I3 q) { q.z = pidx(q.z, s.nz); if (s.perx > .5f) q.x = pidx(q.x, s.nx); return q; }
@@ -25,17 +25,17 @@ __device__ __forceinline__ float old_face(const Ctx& s, I3 q, int axis) {
 __device__ __forceinline__ float face_courant(const Ctx& s, I3 q, int axis) { return old_face(s, q, axis) * timestep(s) / s.h[axis]; }
 
 // coupled_motion.glslinc:66-85: nearest wall within three layers
-__device__ __noinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int component, float& parity) {
+__device__ __forceinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int component, float& parity) {
   float best = 1e20f; I3 selected = q; parity = 1;
   for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
     for (int layer = 1; layer <= 3; layer++) {
-      I3 edge = q; comp(edge, axis) += sign * layer; edge = wrap_face(s, edge);
+      I3 edge = q; addc(edge, axis, sign * layer); edge = wrap_face(s, edge);
       if (!face_in_bounds(s, edge, component)) break;
-      I3 left = edge; comp(left, component)--;
+      I3 left = edge; addc(left, component, -1);
       bool sl = solid(s, left), sr = solid(s, edge);
       bool boundary = axis == component ? (sl != sr) : (!sl && !sr);
       if (!boundary) continue;
-      I3 mirror = q; comp(mirror, axis) += sign * (2 * layer - (axis == component ? 0 : 1)); mirror = wrap_face(s, mirror);
+      I3 mirror = q; addc(mirror, axis, sign * (2 * layer - (axis == component ? 0 : 1))); mirror = wrap_face(s, mirror);
       float distance = ((float)layer - (axis == component ? 0 : .5f)) * s.h[axis];
       if (face_in_bounds(s, mirror, component) && !wall_face(s, mirror, component) && distance < best) {
         selected = mirror; best = distance; parity = axis == component ? -1 : 1;
@@ -57,7 +57,7 @@ __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bo
     valid = true; return parity * coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1], axis);
   }
   if (!face_in_bounds(s, q, axis)) { valid = false; return 0; }
-  I3 l = q; comp(l, axis)--; bool left = solid(s, l), right = solid(s, q);
+  I3 l = q; addc(l, axis, -1); bool left = solid(s, l), right = solid(s, q);
   if (left != right) { valid = true; return 0; }
   if (left && right) {
     float reflection;
@@ -69,15 +69,15 @@ __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bo
   return parity * (forward ? s.motion[forward_base(s) + f] : getv(s, acc, face_index(s, q, axis)));
 }
 // coupled_motion.glslinc:112-137: staggered trilinear sample in fma difference form
-__device__ __noinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+__device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
   F3 lattice = mkf(x.x / s.h[0] - .5f, x.y / s.h[1] - .5f, x.z / s.h[2] - .5f);
-  comp(lattice, axis) += .5f;
+  addc(lattice, axis, .5f);
   I3 lo = mk((int)floorf(lattice.x), (int)floorf(lattice.y), (int)floorf(lattice.z));
   F3 t = mkf(fract(lattice.x), fract(lattice.y), fract(lattice.z));
   lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
   bool interior = (s.flags & 8) != 0 && lo.x >= 1 && lo.y >= 1 && lo.z >= 1 && lo.x + 1 < s.nx && lo.y + 1 < s.ny && lo.z + 1 < s.nz &&
                   !solid(s, mk(lo.x + 1, lo.y - 1, lo.z));
-  I3 fn = dims(s); comp(fn, axis)++;
+  I3 fn = dims(s); addc(fn, axis, 1);
   int acc = accepted(s);
   float values[8];
 #pragma unroll
@@ -102,7 +102,7 @@ __device__ __noinline__ float sampled_component(const Ctx& s, F3 x, int axis, bo
 }
 __device__ __forceinline__ F3 sampled_velocity_checked(const Ctx& s, F3 x, bool& supported) {
   F3 v; supported = true;
-  for (int axis = 0; axis < 3; axis++) { bool ok; float a, b; comp(v, axis) = sampled_component(s, x, axis, false, ok, a, b); supported = supported && ok; }
+  for (int axis = 0; axis < 3; axis++) { bool ok; float a, b; setc(v, axis, sampled_component(s, x, axis, false, ok, a, b)); supported = supported && ok; }
   return v;
 }
 __device__ __forceinline__ float checked_endpoint(const Ctx& s, F3 x, int component, bool forward, bool& supported, float& l0, float& l1) {
@@ -123,7 +123,7 @@ __device__ __forceinline__ F3 trace_rk2(const Ctx& s, F3 x, float dt) {
   u = sampled_velocity_checked(s, mid, ok);
   return mkf(x.x - dt * u.x, x.y - dt * u.y, x.z - dt * u.z);
 }
-__device__ __noinline__ bool trace_supported(const Ctx& s, F3 x, float dt) {
+__device__ __forceinline__ bool trace_supported(const Ctx& s, F3 x, float dt) {
   bool ok; F3 u = sampled_velocity_checked(s, x, ok);
   F3 mid = mkf(x.x - .5f * dt * u.x, x.y - .5f * dt * u.y, x.z - .5f * dt * u.z);
   u = sampled_velocity_checked(s, mid, ok);
@@ -139,16 +139,16 @@ __device__ __noinline__ bool trace_supported(const Ctx& s, F3 x, float dt) {
 __device__ __forceinline__ float courant_at(const Ctx& s, I3 q, int bank, bool accelerate) {
   F3 speed;
   for (int axis = 0; axis < 3; axis++) {
-    I3 r = q; comp(r, axis)++;
-    comp(speed, axis) = gmax(fabsf(getv(s, bank, face_index(s, q, axis))), fabsf(getv(s, bank, face_index(s, r, axis))));
+    I3 r = q; addc(r, axis, 1);
+    setc(speed, axis, gmax(fabsf(getv(s, bank, face_index(s, q, axis))), fabsf(getv(s, bank, face_index(s, r, axis)))));
   }
   if (accelerate && coastal_rate(s, center(s, q).x) > 0) {
     float se, ss; coastal_surface(s, center(s, q).x, s.c[1] + s.dt_max, se, ss);
     if (center(s, q).y < se + 3 * s.h[1]) {
       for (int axis = 0; axis < 3; axis++) {
-        I3 r = q; comp(r, axis)++;
-        comp(speed, axis) = gmax(compc(speed, axis), gmax(fabsf(coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1] + s.dt_max, axis)),
-                                                          fabsf(coastal_velocity_axis(s, coastal_face_position(s, r, axis), s.c[1] + s.dt_max, axis))));
+        I3 r = q; addc(r, axis, 1);
+        setc(speed, axis, gmax(compc(speed, axis), gmax(fabsf(coastal_velocity_axis(s, coastal_face_position(s, q, axis), s.c[1] + s.dt_max, axis)),
+                                                          fabsf(coastal_velocity_axis(s, coastal_face_position(s, r, axis), s.c[1] + s.dt_max, axis)))));
       }
     }
   }
@@ -164,22 +164,22 @@ __device__ __forceinline__ float linear_streamfunction(const Ctx& s, F3 x) {
 }
 __device__ __forceinline__ bool projected_face(const Ctx& s, I3 q, int axis) {
   q = wrap_face(s, q); if (!face_in_bounds(s, q, axis) || wall_face(s, q, axis)) return false;
-  I3 l = q; comp(l, axis)--; return wet(s, q) || wet(s, l);
+  I3 l = q; addc(l, axis, -1); return wet(s, q) || wet(s, l);
 }
 // coupled_motion.glslinc:212-230 (linear air continuation, numerics.y)
-__device__ __noinline__ float extended_value(const Ctx& s, int64_t source, I3 target) {
+__device__ __forceinline__ float extended_value(const Ctx& s, int64_t source, I3 target) {
   int cb = candidate(s);
   float value = getv(s, cb, U_(s) + source);
   if (s.linear_air < .5f) return value;
   I3 q; int component; unpack_face(s, source, q, component);
   F3 g = mkf(0, 0, 0);
   for (int axis = 0; axis < 3; axis++) {
-    I3 l = q, r = q; comp(l, axis)--; comp(r, axis)++; l = wrap_face(s, l); r = wrap_face(s, r);
+    I3 l = q, r = q; addc(l, axis, -1); addc(r, axis, 1); l = wrap_face(s, l); r = wrap_face(s, r);
     bool left = projected_face(s, l, component), right = projected_face(s, r, component);
     float dl = left ? value - getv(s, cb, face_index(s, l, component)) : 0;
     float dr = right ? getv(s, cb, face_index(s, r, component)) - value : 0;
-    if (left && right) comp(g, axis) = dl * dr > 0 ? gsign(dl) * gmin(fabsf(dl), fabsf(dr)) : 0;
-    else comp(g, axis) = left ? dl : (right ? dr : 0);
+    if (left && right) setc(g, axis, dl * dr > 0 ? gsign(dl) * gmin(fabsf(dl), fabsf(dr)) : 0);
+    else setc(g, axis, left ? dl : (right ? dr : 0));
   }
   F3 off = periodic_delta(s, target, q);
   off = mkf(gclamp(off.x / s.h[0], -1, 1), gclamp(off.y / s.h[1], -1, 1), gclamp(off.z / s.h[2], -1, 1));

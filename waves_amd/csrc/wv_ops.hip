@@ -1,7 +1,7 @@
 // Per-step kernels of the gfx950 solver: initialisation, trial control, split
 // geometric VOF, limited MacCormack, coastal relaxation, geometry, PCG vector
 // ops, projection, extension and acceptance.  Each kernel restates one op (or
-// one op and its single-group finaliser, fused through a last-block ticket) of
+// one op and its single-group finaliser, as a second one-group launch) of
 // water/shaders/compute/{pressure_pass,pressure_iterations,coupled_motion,
 // coastal_pass}.glslinc.  Reductions run in 128-lane groups with wavefront
 // shuffles so that their float sums match the reference's workgroup trees.
@@ -12,7 +12,7 @@
 
 // ---------------------------------------------------------------- init ops --
 // pressure_pass.glslinc:3-19 (op 0)
-__global__ void k_op0(Ctx s) {
+__global__ void __launch_bounds__(256) k_op0(Ctx s) {
   int64_t i = GID();
   if (i < s.stride) { s.bank[0][i] = 0; s.bank[1][i] = 0; }
   if (i < WV_NCTRL) s.c[i] = 0;
@@ -23,7 +23,7 @@ __global__ void k_op0(Ctx s) {
   }
 }
 // coupled_motion.glslinc:234-270 (op 40): fixtures, dt, gravity multiplier
-__global__ void k_op40(Ctx s) {
+__global__ void __launch_bounds__(256) k_op40(Ctx s) {
   int64_t i = GID(); int acc = accepted(s);
   if (i < s.N) {
     I3 q = coord(i, dims(s));
@@ -53,7 +53,7 @@ __global__ void k_op40(Ctx s) {
     if (s.shape == 4 && axis == 0 && !wall_face(s, q, axis)) velocity = (float)q.x * s.h[0] < .3f ? 2 : -2;
     if (s.shape == 5) {
       F3 x = mkf(((float)q.x + .5f) * s.h[0] + s.o[0], ((float)q.y + .5f) * s.h[1] + s.o[1], ((float)q.z + .5f) * s.h[2] + s.o[2]);
-      comp(x, axis) -= .5f * s.h[axis];
+      addc(x, axis, -(.5f * s.h[axis]));
       if (x.y < eta(s, q) + 3 * s.h[1] && !wall_face(s, q, axis)) {
         if (axis == 0) { F3 p = x, m = x; p.y += .5f * s.h[1]; m.y -= .5f * s.h[1]; velocity = (linear_streamfunction(s, p) - linear_streamfunction(s, m)) / s.h[1]; }
         if (axis == 1) { F3 p = x, m = x; p.x += .5f * s.h[0]; m.x -= .5f * s.h[0]; velocity = -(linear_streamfunction(s, p) - linear_streamfunction(s, m)) / s.h[0]; }
@@ -68,7 +68,7 @@ __global__ void k_op40(Ctx s) {
 __device__ __forceinline__ uint32_t wv_hash32(uint32_t x) {
   x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16; return x;
 }
-__global__ void k_perturb(Ctx s) {
+__global__ void __launch_bounds__(256) k_perturb(Ctx s) {
   int64_t i = GID(); if (i >= s.N || s.pert_amp == 0) return;
   I3 q = coord(i, dims(s)); if (solid(s, q)) return;
   uint32_t hsh = wv_hash32(s.seed * 0x9e3779b9U + (uint32_t)q.z * 0x85ebca6bU + 0x632be5abU);
@@ -80,7 +80,7 @@ __global__ void k_perturb(Ctx s) {
 // ------------------------------------------------------- geometry (17,18,16) --
 // MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
 template <int MODE>
-__global__ void k_reconstruct(Ctx s) {
+__global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   if (MODE == 0 && s.c[4] > .5f && s.c[5] < .5f) return;
   if (MODE == 1 && !(s.c[5] > .5f && s.c[7] < .5f)) return;
@@ -97,7 +97,7 @@ __global__ void k_reconstruct(Ctx s) {
   } else s.plane[i] = e;
 }
 // pressure_pass.glslinc:131-137 (op 18)
-__global__ void k_distance(Ctx s) {
+__global__ void __launch_bounds__(256) k_distance(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   if (s.c[4] > .5f && s.c[5] < .5f) return;
   I3 q = coord(i, dims(s));
@@ -105,18 +105,18 @@ __global__ void k_distance(Ctx s) {
   s.geo[i] = make_float2(ph, solid(s, q) ? -1.0f : (ph < 0 ? 1.0f : 0.0f));
 }
 // pressure_pass.glslinc:108-115 (op 16)
-__global__ void k_coefficients(Ctx s) {
+__global__ void __launch_bounds__(256) k_coefficients(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   if (s.c[4] > .5f && s.c[5] < .5f) return;
   I3 q = coord(i, dims(s)); float diag = 0; int face = 0; bool w = wet(s, q);
   for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
-    I3 nb = q; comp(nb, axis) += sign; float k = w ? coefficient(s, q, nb, axis) : 0;
+    I3 nb = q; addc(nb, axis, sign); float k = w ? coefficient(s, q, nb, axis) : 0;
     s.work[(int64_t)(6 + face) * s.N + i] = k; diag += k; face++;
   }
   s.work[12 * s.N + i] = w ? diag : 1.0f;
 }
 // pressure_pass.glslinc:104-107 (op 14, plain-F export; not in the coupled program)
-__global__ void k_export_plain(Ctx s) {
+__global__ void __launch_bounds__(256) k_export_plain(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   float f = getv(s, accepted(s), i);
   s.img_planes[i] = make_float4(0, 1, 0, f); s.img_kinds[i] = f == 0 ? 0 : (f == 1 ? 1 : 2);
@@ -124,18 +124,18 @@ __global__ void k_export_plain(Ctx s) {
 
 // ------------------------------------------------------- trial control (43,1) --
 // coupled_motion.glslinc:351-355 then pressure_pass.glslinc:20-24
-__global__ void k_retry(Ctx s) {
+__global__ void __launch_bounds__(256) k_retry(Ctx s) {
   float* c = s.c;
   bool retry = c[7] == 20 || (c[7] >= 10 && c[7] <= 12);
   if (retry && c[29] < 2) { c[33] = c[7]; c[34] = c[20]; c[30] = c[20] * .5f; c[29]++; c[7] = 0; c[4] = 0; c[5] = 0; }
 }
-__global__ void k_trial_start(Ctx s) {
+__global__ void __launch_bounds__(256) k_trial_start(Ctx s) {
   float* c = s.c; float quota = *s.quota;
   c[5] = 0;
   if (c[7] > 0.5f || (quota >= 0 ? c[2] >= quota : c[1] >= -quota)) return;
   if (c[4] < 0.5f) { c[4] = 1; c[5] = 1; c[6] = 0; c[8] = 0; c[13] = 1; c[32]++; c[57] = 0; c[58] = 0; }
 }
-__global__ void k_inject(Ctx s) { s.c[19] = 1; }  // op 15, fault injection
+__global__ void __launch_bounds__(256) k_inject(Ctx s) { s.c[19] = 1; }  // op 15, fault injection
 
 // Serial in-order sum of the block partials' component k (the reference's lane-0
 // loops, e.g. pressure_pass.glslinc:80): the whole group stages the partials in
@@ -153,15 +153,18 @@ __device__ __forceinline__ void serial_partials(const float4* partial, int64_t g
 }
 
 // ---------------------------------------------------- CFL / dt (20 + 21) -----
-__global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID();
   float rate = i < s.N ? courant_at(s, coord(i, dims(s)), accepted(s), true) : 0;
   float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // coupled_motion.glslinc:362-369: a max is order-free, so reduce it in parallel
-  float m = 0; int64_t G = gridDim.x;
+  float m = 0; 
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) m = gmax(m, s.partial[k].y);
   float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
   if (threadIdx.x == 0) {
@@ -172,14 +175,15 @@ __global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, unsigned* ticket) {
   }
 }
 // ---------------------------------------------- compression audit (45 + 46) --
-__global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     int acc = accepted(s); float dt = timestep(s); float cell = s.h[0] * s.h[1] * s.h[2];
     I3 q = coord(i, dims(s)); float f = getv(s, acc, i), dv = 0, raw = 0;
     for (int axis = 0; axis < 3; axis++) {
-      I3 r = q; comp(r, axis)++;
+      I3 r = q; addc(r, axis, 1);
       dv += (old_face(s, r, axis) - old_face(s, q, axis)) / s.h[axis];
       raw += (getv(s, acc, face_index(s, r, axis)) - getv(s, acc, face_index(s, q, axis))) / s.h[axis];
     }
@@ -187,19 +191,21 @@ __global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, unsigned* ticket)
   }
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<false>(s.partial, G);
   if (threadIdx.x == 0) { float* c = s.c; c[35] = g.x; c[36] = g.y; c[37] = g.z; c[38] = g.w; if (g.y > 0) { c[7] = 22; c[3]++; } }
 }
 // ---------------------------------------------------------- bank copy (22) --
-__global__ void k_bank_copy(Ctx s) {
+__global__ void __launch_bounds__(256) k_bank_copy(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); int acc = accepted(s), cb = 1 - acc;
   if (i < s.stride) putv(s, cb, i, getv(s, acc, i));
   if (i < s.N) s.motion[s.N + i] = getv(s, acc, i) > .5f ? 1 : 0;
 }
 // -------------------------------------------------- geometric flux (24) -----
-__global__ void k_flux(Ctx s) {
+__global__ void __launch_bounds__(256) k_flux(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.NF) return;
   int sweep_axis = ((int)s.c[2] + (int)s.c[21]) % 3;
@@ -207,11 +213,11 @@ __global__ void k_flux(Ctx s) {
   float amount = 0;
   if (axis == sweep_axis && !wall_face(s, q, axis)) {
     float velocity = old_face(s, q, axis), courant = face_courant(s, q, axis);
-    I3 donor = q; if (velocity > 0) comp(donor, axis)--;
+    I3 donor = q; if (velocity > 0) addc(donor, axis, -1);
     donor = wrap_cell(s, donor); float f = transport_f(s, donor), width = fabsf(courant); amount = f * width;
     if (f > 0 && f < 1 && width > 0) {
       float4 eq = donor.x < 0 ? coastal_plane(s, donor, s.c[1]) : s.plane[index3(donor, dims(s))];
-      F3 low = mkf(0, 0, 0), extent = mkf(1, 1, 1); comp(extent, axis) = width; if (velocity > 0) comp(low, axis) = 1 - width;
+      F3 low = mkf(0, 0, 0), extent = mkf(1, 1, 1); setc(extent, axis, width); if (velocity > 0) setc(low, axis, 1 - width);
       amount = width * cube_volume(mkf(eq.x * extent.x, eq.y * extent.y, eq.z * extent.z), eq.w - dot3(mkf(eq.x, eq.y, eq.z), low));
     }
     amount *= gsign(velocity);
@@ -219,8 +225,9 @@ __global__ void k_flux(Ctx s) {
   s.motion[flux_base(s) + i] = amount;
 }
 // ---------------------------------------------- boundary ledgers (55 + 56) --
-__global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     int axis = ((int)s.c[2] + (int)s.c[21]) % 3;
@@ -230,18 +237,21 @@ __global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, unsigned* ticket) {
   }
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<false>(s.partial, G);
   if (threadIdx.x == 0) { s.c[39] += g.x; s.c[65] += g.y; s.c[66] += g.z; }
 }
 // ------------------------------------------------ VOF update (25 + 26) ------
-__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID(); float4 v = make_float4(1e20f, -1e20f, 0, 0);
   int sweep = (int)s.c[21];
   if (i < s.N) {
     int axis = ((int)s.c[2] + sweep) % 3;
-    I3 q = coord(i, dims(s)), r = q; comp(r, axis)++;
+    I3 q = coord(i, dims(s)), r = q; addc(r, axis, 1);
     float net = s.motion[flux_base(s) + face_local(s, q, axis)] - s.motion[flux_base(s) + face_local(s, r, axis)];
     float correction = s.motion[s.N + i] * (face_courant(s, r, axis) - face_courant(s, q, axis));
     float increment = net + correction;
@@ -252,14 +262,16 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, unsigned* ticket) 
   }
   v = group_reduce<RED_BOUNDS>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // coupled_motion.glslinc:402-408: min/max in parallel, the two sums in index order
   float mn = 1e20f, mx = -1e20f;
-  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
   __shared__ float zs, ws;
   if (threadIdx.x == 0) { zs = 0; ws = 0; }
-  serial_partials(s.partial, gridDim.x, [&](float4 p) { zs += p.z; ws += p.w; });
+  serial_partials(s.partial, G, [&](float4 p) { zs += p.z; ws += p.w; });
   if (threadIdx.x == 0) {
     float* c = s.c; float cell = s.h[0] * s.h[1] * s.h[2];
     uint32_t step = (uint32_t)(int64_t)c[32] - 1u, sw = (uint32_t)c[21];
@@ -273,12 +285,12 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, unsigned* ticket) 
 // ---------------------------------------------- MacCormack (27 / 28) --------
 // coupled_motion.glslinc:168-188, 409-426 (transport-cache and plain forms)
 template <bool CORRECT>
-__global__ void k_maccormack(Ctx s) {
+__global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.NF) return;
   I3 q; int axis; unpack_face(s, i, q, axis);
   F3 x = mkf(((float)q.x + .5f) * s.h[0], ((float)q.y + .5f) * s.h[1], ((float)q.z + .5f) * s.h[2]);
-  comp(x, axis) -= .5f * s.h[axis];
+  addc(x, axis, -(.5f * s.h[axis]));
   int acc = accepted(s); float dt = timestep(s);
   float value = 0;
   if (!wall_face(s, q, axis) && ((s.flags & 2) == 0 || s.motion[valid_base(s, acc) + i] > .5f)) {
@@ -324,13 +336,14 @@ __global__ void k_maccormack(Ctx s) {
 // --------------------------------------------------- coastal relaxation -----
 __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
   F3 v;
-  for (int axis = 0; axis < 3; axis++) { I3 r = q; comp(r, axis)++; comp(v, axis) = .5f * (getv(s, bank, face_index(s, q, axis)) + getv(s, bank, face_index(s, r, axis))); }
+  for (int axis = 0; axis < 3; axis++) { I3 r = q; addc(r, axis, 1); setc(v, axis, .5f * (getv(s, bank, face_index(s, q, axis)) + getv(s, bank, face_index(s, r, axis)))); }
   return v;
 }
 __device__ __forceinline__ bool offshore_zone(const Ctx& s, float x) { return x < s.forc[0] + s.forc[1]; }
 // coastal_pass.glslinc:46-58 (ops 80 + 81)
-__global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     I3 q = coord(i, dims(s));
@@ -346,12 +359,14 @@ __global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, unsigned* ticket) {
   }
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<false>(s.partial, G);
   if (threadIdx.x == 0) { s.c[64] += g.x; s.c[67] += g.y; s.c[68] += g.z; s.c[69] += g.w; }
 }
 // coastal_pass.glslinc:59-68 (op 82)
-__global__ void k_relax_faces(Ctx s) {
+__global__ void __launch_bounds__(256) k_relax_faces(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.NF) return;
   I3 q; int axis; unpack_face(s, i, q, axis);
@@ -366,37 +381,43 @@ __global__ void k_relax_faces(Ctx s) {
   putv(s, cb, U_(s) + i, value); s.motion[forward_base(s) + i] = value - old;
 }
 // coastal_pass.glslinc:69-77 (ops 83 + 84)
-__global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     I3 q = coord(i, dims(s)); F3 ch; float cell = s.h[0] * s.h[1] * s.h[2];
-    for (int axis = 0; axis < 3; axis++) { I3 r = q; comp(r, axis)++; comp(ch, axis) = .5f * (s.motion[forward_base(s) + face_local(s, q, axis)] + s.motion[forward_base(s) + face_local(s, r, axis)]); }
+    for (int axis = 0; axis < 3; axis++) { I3 r = q; addc(r, axis, 1); setc(ch, axis, .5f * (s.motion[forward_base(s) + face_local(s, q, axis)] + s.motion[forward_base(s) + face_local(s, r, axis)])); }
     float m = getv(s, candidate(s), i) * 1000 * cell;
     v = make_float4(ch.x * m, ch.y * m, ch.z * m, 0);
   }
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<false>(s.partial, G);
   if (threadIdx.x == 0) { s.c[67] += g.x; s.c[68] += g.y; s.c[69] += g.z; }
 }
 // coastal_pass.glslinc:84-92 (ops 87 + 88)
-__global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  if (!final) {
   int64_t i = GID();
   float f = i < s.N ? getv(s, candidate(s), i) : 0;
   float4 v = (isnan(f) || isinf(f)) ? make_float4(-1e30f, 1e30f, 0, 0) : make_float4(f, f, 0, 0);
   v = group_reduce<RED_BOUNDS>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   float mn = 1, mx = 0;
-  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
   if (threadIdx.x == 0) { s.c[88] = b.x; s.c[89] = b.y; if (b.x < 0 || b.y > 1) { s.c[7] = 23; s.c[3]++; } }
 }
 // coastal_pass.glslinc:78-83 (op 86): eight analytic target probes
-__global__ void k_probes(Ctx s) {
+__global__ void __launch_bounds__(256) k_probes(Ctx s) {
   int i = threadIdx.x; if (i >= 8) return;
   float probe_time = s.forc[3] * 6.28318530718f / s.wave[2] * (i < 4 ? .37f : 1);
   F3 point = mkf(s.o[0] + .17f + (float)i * .23f, -s.wave[3] + (float)i * .07f, 0);
@@ -405,7 +426,7 @@ __global__ void k_probes(Ctx s) {
 }
 // coastal_pass.glslinc:102-113 (op 85): eight column-height gauges; one block
 // per gauge, columns in parallel over z, the z sum in index order
-__global__ void k_gauges(Ctx s) {
+__global__ void __launch_bounds__(256) k_gauges(Ctx s) {
   __shared__ float columns[1024];
   int g = blockIdx.x; int acc = accepted(s);
   float begin = s.forc[0] + s.forc[1] + .4f;
@@ -426,7 +447,7 @@ __global__ void k_gauges(Ctx s) {
   if (threadIdx.x == 0) s.c[80 + g] = height / (float)s.nz;
 }
 // coastal_pass.glslinc:93-101 (op 89)
-__global__ void k_coastal_history(Ctx s) {
+__global__ void __launch_bounds__(256) k_coastal_history(Ctx s) {
   float* c = s.c;
   if (c[2] <= c[91] || c[2] > s.hist_cap) return;
   int64_t row = ((int64_t)c[2] - 1) * 5;
@@ -438,7 +459,7 @@ __global__ void k_coastal_history(Ctx s) {
   c[91] = c[2];
 }
 // coupled_motion.glslinc:338-340 (op 44)
-__global__ void k_history(Ctx s) {
+__global__ void __launch_bounds__(256) k_history(Ctx s) {
   float* c = s.c;
   if (c[32] > 0 && c[32] <= s.hist_cap) {
     int64_t row = ((int64_t)c[32] - 1) * 4 + 3;
@@ -448,7 +469,7 @@ __global__ void k_history(Ctx s) {
 
 // ------------------------------------------- pressure preparation (2, 47) ---
 // pressure_pass.glslinc:25-50 (op 2)
-__global__ void k_gravity_prep(Ctx s) {
+__global__ void __launch_bounds__(256) k_gravity_prep(Ctx s) {
   if (s.c[5] < 0.5f) return;
   int64_t i = GID(); if (i >= s.stride) return;
   int acc = accepted(s), cb = 1 - acc; float dt = timestep(s);
@@ -458,15 +479,15 @@ __global__ void k_gravity_prep(Ctx s) {
   if (i >= P) val = wet(s, coord(i - P, dims(s))) ? getv(s, acc, i) : 0;
   if (i >= Ub && i < P) {
     int axis = i < Vb ? 0 : (i < Wb ? 1 : 2); int64_t base = axis == 0 ? Ub : (axis == 1 ? Vb : Wb);
-    I3 fn = dims(s); comp(fn, axis)++;
-    I3 r = coord(i - base, fn), l = r; comp(l, axis)--;
+    I3 fn = dims(s); addc(fn, axis, 1);
+    I3 r = coord(i - base, fn), l = r; addc(l, axis, -1);
     if (solid(s, l) || solid(s, r) || (!wet(s, l) && !wet(s, r))) val = 0.0f;
     else if (axis == 1) val -= 9.81f * dt * s.c[31];
   }
   putv(s, cb, i, val);
 }
 // coupled_motion.glslinc:319-322 (op 47)
-__global__ void k_periodic_sync(Ctx s) {
+__global__ void __launch_bounds__(256) k_periodic_sync(Ctx s) {
   if (s.c[4] < .5f || s.c[7] > .5f) return;
   int64_t i = GID(); if (i >= s.NF) return;
   I3 q; int axis; unpack_face(s, i, q, axis);
@@ -474,13 +495,13 @@ __global__ void k_periodic_sync(Ctx s) {
   if (canonical != U_(s) + i) { int cb = candidate(s); putv(s, cb, U_(s) + i, getv(s, cb, canonical)); }
 }
 // coupled_motion.glslinc:282-284 (ops 63, 64, 65)
-__global__ void k_defect_state(Ctx s) { float* c = s.c; if (c[57] == 1 && c[7] < .5f) { c[57] = 2; c[6] = 0; c[8] = 0; } }
-__global__ void k_defect_save(Ctx s) {
+__global__ void __launch_bounds__(256) k_defect_state(Ctx s) { float* c = s.c; if (c[57] == 1 && c[7] < .5f) { c[57] = 2; c[6] = 0; c[8] = 0; } }
+__global__ void __launch_bounds__(256) k_defect_save(Ctx s) {
   if (!(s.c[57] == 2 && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.N) return;
   int cb = candidate(s); s.work[13 * s.N + i] = getv(s, cb, P_(s) + i); putv(s, cb, P_(s) + i, 0);
 }
-__global__ void k_defect_restore(Ctx s) {
+__global__ void __launch_bounds__(256) k_defect_restore(Ctx s) {
   if (!(s.c[4] > .5f && s.c[6] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.N) return;
   int cb = candidate(s); putv(s, cb, P_(s) + i, getv(s, cb, P_(s) + i) + s.work[13 * s.N + i]);
@@ -497,7 +518,7 @@ __device__ __forceinline__ float operator_at(const Ctx& s, I3 q, int which) {
   float result = 0, here = vector_at(s, q, which); int64_t i = index3(q, dims(s));
   int face = 0;
   for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
-    I3 nb = q; comp(nb, axis) += sign; float k = s.work[(int64_t)(6 + face) * s.N + i];
+    I3 nb = q; addc(nb, axis, sign); float k = s.work[(int64_t)(6 + face) * s.N + i];
     result += k * (here - vector_at(s, nb, which)); face++;
   }
   return result;
@@ -505,7 +526,7 @@ __device__ __forceinline__ float operator_at(const Ctx& s, I3 q, int which) {
 // pressure_iterations.glslinc:4-13 (op 3).  The block partials of op 3 are
 // replaced by the V-cycle's output partials before op 4 reads them
 // (pressure_multigrid.glslinc:230-234), so this kernel publishes none.
-__global__ void k_pcg_init(Ctx s) {
+__global__ void __launch_bounds__(256) k_pcg_init(Ctx s) {
   if (!pressure_setup(s)) return;
   int64_t i = GID(); if (i >= s.N) return;
   I3 q = coord(i, dims(s)); float r = 0, z = 0, bv = 0, diag = 1; float dt = timestep(s);
@@ -514,20 +535,23 @@ __global__ void k_pcg_init(Ctx s) {
   s.work[i] = r; s.work[N + i] = z; s.work[2 * N + i] = z; s.work[4 * N + i] = diag; s.work[5 * N + i] = bv;
 }
 // pressure_iterations.glslinc:20 (op 5)
-__global__ void k_spmv(Ctx s) {
+__global__ void __launch_bounds__(256) k_spmv(Ctx s) {
   if (!solving(s)) return;
   int64_t i = GID(); if (i >= s.N) return;
   I3 q = coord(i, dims(s)); s.work[3 * s.N + i] = wet(s, q) ? operator_at(s, q, 1) : 0;
 }
 // pressure_iterations.glslinc:21-22 (ops 6 + 7)
-__global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, int final) {
   if (!solving(s)) return;
+  if (!final) {
   int64_t i = GID(); int64_t N = s.N;
   float4 v = i < N ? make_float4(s.work[2 * N + i] * s.work[3 * N + i], 0, 0, 0) : make_float4(0, 0, 0, 0);
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float dotp = global_reduce<false>(s.partial, gridDim.x).x;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float dotp = global_reduce<false>(s.partial, G).x;
   if (threadIdx.x == 0) {
     if (dotp <= 0 || isnan(dotp) || isinf(dotp)) { s.c[7] = 1; return; }
     s.c[11] = s.c[9] / dotp;
@@ -536,7 +560,7 @@ __global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, unsigned* ticket) {
 // pressure_iterations.glslinc:23-30 (op 8) fused with the V-cycle's load of r
 // (pressure_multigrid.glslinc:191).  z = r/diag and the op-8 partials are
 // superseded by the V-cycle output and are not written.
-__global__ void k_pcg_update(Ctx s, float* mg_b, float* mg_x1, float* mg_x2) {
+__global__ void __launch_bounds__(256) k_pcg_update(Ctx s, float* mg_b, float* mg_x1, float* mg_x2) {
   if (!solving(s)) return;
   int64_t i = GID(); int64_t N = s.N; if (i >= N) return;
   float alpha = s.c[11];
@@ -547,20 +571,20 @@ __global__ void k_pcg_update(Ctx s, float* mg_b, float* mg_x1, float* mg_x2) {
   mg_b[i] = r; mg_x1[i] = 0; mg_x2[i] = 0;
 }
 // pressure_iterations.glslinc:32 (op 10)
-__global__ void k_direction(Ctx s) {
+__global__ void __launch_bounds__(256) k_direction(Ctx s) {
   if (!solving(s)) return;
   int64_t i = GID(); int64_t N = s.N; if (i >= N) return;
   s.work[2 * N + i] = s.work[N + i] + s.c[12] * s.work[2 * N + i];
 }
 
 // --------------------------------------------------------- projection (11) --
-__global__ void k_project(Ctx s) {
+__global__ void __launch_bounds__(256) k_project(Ctx s) {
   if (s.c[4] < 0.5f || s.c[6] < 0.5f || s.c[7] > 0.5f) return;
   int64_t i = U_(s) + GID(); int64_t P = P_(s); if (i >= P) return;
   int64_t Vb = V_(s), Wb = W_(s);
   int axis = i < Vb ? 0 : (i < Wb ? 1 : 2); int64_t base = axis == 0 ? U_(s) : (axis == 1 ? Vb : Wb);
-  I3 fn = dims(s); comp(fn, axis)++;
-  I3 r = coord(i - base, fn), l = r; comp(l, axis)--;
+  I3 fn = dims(s); addc(fn, axis, 1);
+  I3 r = coord(i - base, fn), l = r; addc(l, axis, -1);
   int cb = candidate(s); float dt = timestep(s);
   float value = 0;
   if (!solid(s, l) && !solid(s, r) && (wet(s, l) || wet(s, r)))
@@ -570,13 +594,13 @@ __global__ void k_project(Ctx s) {
 }
 // ---------------------------------------------------- extension (30 - 34) --
 __device__ __forceinline__ bool post_ok(const Ctx& s) { return !(s.c[4] < .5f || s.c[6] < .5f || s.c[7] > .5f); }
-__global__ void k_ext_seed(Ctx s) {
+__global__ void __launch_bounds__(256) k_ext_seed(Ctx s) {
   if (!post_ok(s)) return;
   int64_t i = GID(); if (i >= s.NF) return;
-  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; addc(l, axis, -1);
   s.motion[source_base(s, 0) + i] = !wall_face(s, q, axis) && (wet(s, q) || wet(s, l)) ? (float)i : -1;
 }
-__global__ void k_ext_hop(Ctx s, int rb) {
+__global__ void __launch_bounds__(256) k_ext_hop(Ctx s, int rb) {
   if (!post_ok(s)) return;
   int64_t i = GID(); if (i >= s.NF) return;
   int wb = 1 - rb;
@@ -585,7 +609,7 @@ __global__ void k_ext_hop(Ctx s, int rb) {
   if (!wall_face(s, q, axis)) {
     if (source >= 0) { I3 sq; int sa; unpack_face(s, (int64_t)source, sq, sa); F3 d = periodic_delta(s, q, sq); best = dot3(d, d); }
     for (int ax = 0; ax < 3; ax++) for (int sign = -1; sign <= 1; sign += 2) {
-      I3 nb = q; comp(nb, ax) += sign; nb = wrap_face(s, nb);
+      I3 nb = q; addc(nb, ax, sign); nb = wrap_face(s, nb);
       if (!face_in_bounds(s, nb, axis) || wall_face(s, nb, axis)) continue;
       float cs = s.motion[source_base(s, rb) + face_local(s, nb, axis)]; if (cs < 0) continue;
       I3 sq; int sa; unpack_face(s, (int64_t)cs, sq, sa); F3 d = periodic_delta(s, q, sq); float distance = dot3(d, d);
@@ -594,10 +618,10 @@ __global__ void k_ext_hop(Ctx s, int rb) {
   } else source = -1;
   s.motion[source_base(s, wb) + i] = source;
 }
-__global__ void k_ext_fill(Ctx s) {
+__global__ void __launch_bounds__(256) k_ext_fill(Ctx s) {
   if (!post_ok(s)) return;
   int64_t i = GID(); if (i >= s.NF) return;
-  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+  I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; addc(l, axis, -1);
   if (wet(s, q) || wet(s, l)) return;
   float source = s.motion[source_base(s, 1) + i];
   float value = source < 0 ? 0 : extended_value(s, (int64_t)source, q);
@@ -605,7 +629,7 @@ __global__ void k_ext_fill(Ctx s) {
 }
 // coupled_motion.glslinc:323-337 (ops 48 / 49)
 template <bool INITIAL>
-__global__ void k_masks(Ctx s) {
+__global__ void __launch_bounds__(256) k_masks(Ctx s) {
   if (!INITIAL && !post_ok(s)) return;
   int64_t i = GID(); int cb = candidate(s);
   if (i < s.N) {
@@ -614,7 +638,7 @@ __global__ void k_masks(Ctx s) {
     else s.motion[pmask_base(s, cb) + i] = flag;
   }
   if (i < s.NF) {
-    I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; comp(l, axis)--;
+    I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; addc(l, axis, -1);
     float valid = INITIAL ? (!wall_face(s, q, axis) && (wet(s, q) || wet(s, l) || center(s, q).y < eta(s, q) + 3 * s.h[1]) ? 1 : 0)
                           : (s.motion[source_base(s, 1) + i] >= 0 ? 1 : 0);
     if (INITIAL) { s.motion[valid_base(s, 0) + i] = valid; s.motion[valid_base(s, 1) + i] = valid; }
@@ -622,27 +646,33 @@ __global__ void k_masks(Ctx s) {
   }
 }
 // --------------------------------------------- post-projection CFL (35+36) --
-__global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, int final) {
   if (!post_ok(s)) return;
+  if (!final) {
   int64_t i = GID();
   float rate = i < s.N ? courant_at(s, coord(i, dims(s)), candidate(s), false) * timestep(s) : 0;
   float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   float m = 0;
-  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) m = gmax(m, s.partial[k].y);
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) m = gmax(m, s.partial[k].y);
   float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
   if (threadIdx.x == 0) s.c[27] = r.x;
 }
 // ------------------------------------------------ true residual (62 + 66) --
-__global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, int final) {
   if (!post_ok(s)) return;
+  if (!final) {
   int64_t i = GID(); float r = 0;
   if (i < s.N) { I3 q = coord(i, dims(s)); if (wet(s, q)) r = 1000 / timestep(s) * divergence(s, q); }
   float4 v = group_reduce<RED_SUM>(make_float4(r * r, 0, 0, 0));
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float r2 = global_reduce<false>(s.partial, gridDim.x).x;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float r2 = global_reduce<false>(s.partial, G).x;
   if (threadIdx.x == 0) {
     float* c = s.c;
     c[60] = sqrtf(r2 / gmax(c[59], 1e-30f)); c[62] = pressure_scaled_norm(s, c[59]); c[63] = pressure_scaled_norm(s, r2);
@@ -650,13 +680,14 @@ __global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, unsigned* ticke
   }
 }
 // -------------------------------------- diagnostics + acceptance (12 + 13) --
-__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int final) {
   if (!post_ok(s)) return;
+  if (!final) {
   int64_t i = GID(); float4 value = make_float4(0, 0, 0, 0); float dt = timestep(s);
   if (i < s.N) {
     int cb = candidate(s); I3 q = coord(i, dims(s)); float speed = 0;
     for (int axis = 0; axis < 3; axis++) {
-      I3 r = q; comp(r, axis)++;
+      I3 r = q; addc(r, axis, 1);
       speed = gmax(speed, gmax(fabsf(getv(s, cb, face_index(s, q, axis))), fabsf(getv(s, cb, face_index(s, r, axis)))));
     }
     bool w = wet(s, q);
@@ -668,16 +699,18 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, unsigned* ticket) {
   }
   value = group_reduce<RED_DIAG>(value);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = value;
-  if (!last_block(ticket)) return;
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // pressure_pass.glslinc:78-103: maxima in parallel, the volume in index order
   float4 mx = make_float4(0, 0, 0, 0);
-  for (int64_t k = threadIdx.x; k < (int64_t)gridDim.x; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
   mx = group_reduce<RED_DIAG>(mx);
   __shared__ float vol;
   if (threadIdx.x == 0) vol = 0;
-  serial_partials(s.partial, gridDim.x, [&](float4 p) { vol += p.w; });
+  serial_partials(s.partial, G, [&](float4 p) { vol += p.w; });
   if (threadIdx.x != 0) return;
-  float* c = s.c; float4 v = make_float4(mx.x, mx.y, mx.z, vol);
+  float* c = s.c; float4 v = make_float4(mx.x, mx.y, mx.z, vol); float dt = timestep(s);
   c[14] = v.x; c[15] = v.y; c[16] = v.z; c[17] = v.w;
   if ((v.x >= 1e-4f || c[61] < .5f) && v.y < 1e20f && c[19] < .5f && c[58] < 4) { c[57] = 1; c[58]++; c[6] = 0; return; }
   if (c[61] < .5f) { c[3]++; c[7] = 4; return; }
@@ -692,20 +725,26 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, unsigned* ticket) {
   c[0] = (float)candidate(s); c[1] += dt; c[2]++; c[4] = 0;
 }
 // ------------------------------------------------ initial volume (41 + 42) --
-__global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, unsigned* ticket) {
+__global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, int final) {
+  if (!final) {
   int64_t i = GID();
   float4 v = i < s.N ? make_float4(getv(s, accepted(s), i) * s.h[0] * s.h[1] * s.h[2], 0, 0, 0) : make_float4(0, 0, 0, 0);
   v = group_reduce<RED_SUM>(v);
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
-  if (!last_block(ticket)) return;
-  float4 g = global_reduce<false>(s.partial, gridDim.x);
+  return;
+  }
+  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  float4 g = global_reduce<false>(s.partial, G);
   if (threadIdx.x == 0) s.c[26] = g.x;
 }
 
 // ------------------------------------------------------------- launchers ----
 static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 #define L256(k, n) hipLaunchKernelGGL(k, dim3(blocks(n, 256)), dim3(256), 0, st, s)
-#define LRED(k, id) hipLaunchKernelGGL(k, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, s.counters + (id))
+// a reduction op: per-group partials over all cells, then one group finalises
+// (a separate launch: the kernel boundary publishes the partials to every XCD)
+#define LRED(k, id) do { hipLaunchKernelGGL(k, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0); \
+                         hipLaunchKernelGGL(k, dim3(1), dim3(WV_RED), 0, st, s, 1); } while (0)
 #define L1(k) hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, st, s)
 
 int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
@@ -770,7 +809,7 @@ void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStre
   hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, b0, x1, x2);
 }
 // masked 7-point SpMV launched on its own for live timing (writes only Ap)
-__global__ void k_spmv_timed(Ctx s) {
+__global__ void __launch_bounds__(256) k_spmv_timed(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   I3 q = coord(i, dims(s)); s.work[3 * s.N + i] = wet(s, q) ? operator_at(s, q, 1) : 0;
 }

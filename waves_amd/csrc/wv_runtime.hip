@@ -214,7 +214,6 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.img_planes, c.N);
   ALLOC(c.img_kinds, c.N);
   ALLOC(c.partial, (c.stride + WV_RED - 1) / WV_RED + 1);
-  ALLOC(c.counters, 64);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
   MGDev& m = s->mg;
