@@ -14,15 +14,20 @@ struct MGLevel {
   int off_start;   // first stencil offset (index into MGDev::offsets / 3)
   int64_t vec_off; // first float of this level's 8 vector fields
   int64_t row_off; // first float of this level's k-major rows (levels >= 1)
+  int margin[3];   // largest |offset| per axis of the stencil: rows this far from every edge are interior
+  int lanes;       // lanes cooperating on one row in the coarse-level kernels (power of two <= 32)
+  int box[3];      // extent of one fine row's (A P) box over coarse nodes (Galerkin stage 1)
 };
 struct MGDev {
   int L, perx;
   MGLevel lv[WV_MAXL];
   const int* offsets;        // stencil offsets, 3 ints each
   const signed char* slots;  // per level: 125 delta slots -> stencil index, -1 if absent
+  const int* deltas;         // per stencil entry: linear index offset for interior rows
   float* vec;                // node_count * 8 floats: b, x1, x2, active, l1 D^-1, -, r, d
   float* rows;               // coarse operator rows, k-major, FP32
   float* linv;               // last level: dense L^-1 (count^2)
+  float* ap;                 // Galerkin scratch: one fine level's (A P) rows, box-major
   float* snapshot;           // 9 N: preconditioner's fine coefficients, kind, face signature
   float* previous;           // 9 N: previous trial's operator audit copy
   unsigned* state;           // hierarchy audit header + per-trial records

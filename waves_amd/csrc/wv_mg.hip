@@ -106,6 +106,112 @@ __global__ void __launch_bounds__(256) k_mg_cheb(Ctx s, MGDev m, int l, int sour
   float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + d;
 }
+// ---- coarse levels (l >= 1): G lanes per row --------------------------------
+// Small levels cannot fill the chip with one thread per row, and a 125-wide row
+// walked by one thread is a chain of dependent loads.  Here G lanes share a row:
+// lane t takes stencil entries t, t+G, ..., then a butterfly over the G lanes
+// sums the partials (a fixed order, so results are deterministic).  Interior
+// rows address neighbours by a precomputed linear delta; rows near an edge go
+// through the periodic/clamped wrap of pressure_multigrid.glslinc:36-41.
+struct RowCtx { int row, lane, G; bool valid; };
+__device__ __forceinline__ RowCtx row_ctx(int G, int N) {
+  RowCtx r; int64_t gid = GID(); r.G = G; r.row = (int)(gid / G); r.lane = (int)(gid % G); r.valid = r.row < N; return r;
+}
+__device__ __forceinline__ float group_sum(float v, int G) {
+  for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// sum_k A[row, k] v_field[col_k]; every lane returns the row total
+__device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
+  const MGLevel& L = m.lv[l]; int N = L.count, W = L.width;
+  float sum = 0;
+  if (rc.valid) {
+    I3 q = coord(rc.row, mdims(m, l));
+    bool interior = q.x >= L.margin[0] && q.x < L.n[0] - L.margin[0] && q.y >= L.margin[1] && q.y < L.n[1] - L.margin[1] &&
+                    q.z >= L.margin[2] && q.z < L.n[2] - L.margin[2];
+    const float* row = m.rows + L.row_off + rc.row;
+    const float* v = m.vec + L.vec_off + (int64_t)field * N;
+    if (interior) {
+      for (int k = rc.lane; k < W; k += rc.G) sum += row[(int64_t)k * N] * v[rc.row + sh_delta[k]];
+    } else {
+      for (int k = rc.lane; k < W; k += rc.G) {
+        int j = mwrap(m, l, mk(q.x + sh_off[3 * k], q.y + sh_off[3 * k + 1], q.z + sh_off[3 * k + 2]));
+        sum += row[(int64_t)k * N] * v[j >= 0 ? j : 0];
+      }
+    }
+  }
+  return group_sum(sum, rc.G);
+}
+__device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off, int* sh_delta) {
+  const MGLevel& L = m.lv[l];
+  for (int t = threadIdx.x; t < L.width; t += blockDim.x) {
+    const int* o = m.offsets + 3 * (L.off_start + t);
+    sh_off[3 * t] = o[0]; sh_off[3 * t + 1] = o[1]; sh_off[3 * t + 2] = o[2]; sh_delta[t] = m.deltas[L.off_start + t];
+  }
+  __syncthreads();
+}
+// OP 0: Chebyshev sweep k from `source`; 1: residual of `source`; 2: Jacobi sweep from `source`
+template <int OP>
+__global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  __shared__ int sh_off[3 * 125], sh_delta[125];
+  stage_stencil(m, l, sh_off, sh_delta);
+  const MGLevel& L = m.lv[l];
+  RowCtx rc = row_ctx(L.lanes, L.count);
+  float ax = coarse_apply(m, l, source, rc, sh_off, sh_delta);
+  if (!rc.valid || rc.lane != 0) return;
+  int i = rc.row;
+  float xs = m.vec[at(m, l, source, i)];
+  if (!mactive(m, l, i)) ax = xs;  // inactive rows are the identity (pressure_multigrid.glslinc:91)
+  float r = m.vec[at(m, l, 0, i)] - ax;
+  if (OP == 1) { m.vec[at(m, l, 6, i)] = r; return; }
+  if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = xs + m.vec[at(m, l, 4, i)] * r; return; }
+  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
+  m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = xs + d;
+}
+// op 8 into level l (from l-1) with G lanes per coarse node over the fine support
+__global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  const MGLevel& C = m.lv[l]; int parent = l - 1; const MGLevel& P = m.lv[parent];
+  RowCtx rc = row_ctx(C.lanes, C.count);
+  float sum = 0;
+  I3 nf = mdims(m, parent), nl = mdims(m, l);
+  if (rc.valid && mactive(m, l, rc.row)) {
+    I3 qc = coord(rc.row, nl);
+    int lo[3], ext[3] = {1, 1, 1}, nfa[3] = {nf.x, nf.y, nf.z}, nla[3] = {nl.x, nl.y, nl.z}, qca[3] = {qc.x, qc.y, qc.z};
+    for (int a = 0; a < 3; a++) lo[a] = qca[a] * P.f[a];
+    if (P.aggregate) { ext[0] = P.f[0]; ext[1] = P.f[1]; ext[2] = P.f[2]; }
+    else for (int a = 0; a < 3; a++) if (P.f[a] > 1) {
+      lo[a]--; ext[a] = 4;
+      if ((a == 2 || (a == 0 && m.perx)) && nfa[a] <= 4) { lo[a] = 0; ext[a] = nfa[a]; }
+    }
+    int total = ext[0] * ext[1] * ext[2];
+    for (int t = rc.lane; t < total; t += rc.G) {
+      int x = t % ext[0], y = (t / ext[0]) % ext[1], z = t / (ext[0] * ext[1]);
+      int j = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)); if (!mactive(m, parent, j)) continue;
+      float weight = 1;
+      if (!P.aggregate) {
+        I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; a++) {
+          float pos = ((float)qa[a] + .5f) / (float)P.f[a] - .5f; int base = (int)floorf(pos); float tt = fract(pos);
+          float w = 0;
+          for (int bit = 0; bit < P.f[a]; bit++) {
+            int nb = base + bit;
+            if (a == 2 || (a == 0 && m.perx)) nb = pmod(nb, nla[a]);
+            else if (a == 0) nb = iclamp(nb, 0, nla[a] - 1); else nb = nb > 0 ? nb : 0;
+            if (nb == qca[a]) w += P.f[a] == 1 ? 1 : (bit == 1 ? tt : 1 - tt);
+          }
+          weight *= w;
+        }
+      }
+      sum += weight * m.vec[at(m, parent, 6, j)];
+    }
+  }
+  sum = group_sum(sum, rc.G);
+  if (!rc.valid || rc.lane != 0) return;
+  m.vec[at(m, l, 0, rc.row)] = sum; m.vec[at(m, l, 1, rc.row)] = 0; m.vec[at(m, l, 2, rc.row)] = 0;
+}
+
 // terminal smoother when coarsening stops above 64 nodes
 // (water/pressure_multigrid.gd:144-149): op 14 then op 6 sweeps
 __global__ void __launch_bounds__(256) k_mg_jacobi(Ctx s, MGDev m, int l, int source, int first, int phase) {
@@ -354,18 +460,88 @@ __global__ void __launch_bounds__(256) k_mg_l1_coarse(Ctx s, MGDev m, int l) {
   m.vec[at(m, l, 4, i)] = 1 / sum;
 }
 
-#define GAL_THREADS 64
-// Galerkin rows of level l+1 from level l: row I gathers
-//   C[I, J] = sum_i sum_j P[i, I] A[i, j] P[j, J]
-// over the fine nodes i whose transfers reach I.  Accumulators live in LDS.
-__global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin(Ctx s, MGDev m, int l) {
+// ---- Galerkin P^T A P in two gather stages ----------------------------------
+// Stage 1, one thread per fine node i of level l: the row (A P)_i, i.e.
+//   T_i[J] = sum_j A[i, j] P[j, J]
+// kept as a small dense box over the coarse nodes J the row can reach, indexed
+// by the parents' unclamped coordinates relative to an anchor (so the box is a
+// fixed shape per level).  Stage 2, G lanes per coarse node I of level l+1:
+//   C[I, J] = sum_i P[i, I] T_i[J]
+// over the fine nodes i whose transfers reach I.  Both stages run their sums in
+// a fixed order: the coarse operator is deterministic.
+__device__ __forceinline__ int floordiv(int a, int b) { int q = a / b; return (a % b != 0 && (a < 0) != (b < 0)) ? q - 1 : q; }
+// first (unclamped) parent coordinate of fine coordinate j along axis a
+__device__ __forceinline__ int parent_base(const MGLevel& F, int a, int j) {
+  if (F.aggregate) return floordiv(j, F.f[a]);
+  if (F.f[a] == 1) return j;
+  return (int)floorf(((float)j + .5f) / (float)F.f[a] - .5f);
+}
+// coarse node of an unclamped parent coordinate (pressure_multigrid.glslinc:115-117),
+// -1 if dropped (beyond the top)
+__device__ __forceinline__ int parent_node(const MGDev& m, int lc, const int P[3]) {
+  I3 n = mdims(m, lc);
+  int y = P[1]; if (y >= n.y) return -1; if (y < 0) y = 0;
+  int x = m.perx ? pmod(P[0], n.x) : iclamp(P[0], 0, n.x - 1);
+  int z = pmod(P[2], n.z);
+  return (int)index3(mk(x, y, z), n);
+}
+__global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
-  extern __shared__ float acc[];  // width(l+1) * GAL_THREADS
+  __shared__ float acc[64 * 128];
+  const MGLevel& F = m.lv[l]; int lc = l + 1;
+  int64_t i = GID(); int tid = threadIdx.x;
+  int E0 = F.box[0], E1 = F.box[1], E = F.box[0] * F.box[1] * F.box[2];
+  for (int b = 0; b < E; b++) acc[b * 128 + tid] = 0;
+  if (i >= F.count) return;
+  if (!mactive(m, l, (int)i)) return;
+  I3 q = coord(i, mdims(m, l)); int qa[3] = {q.x, q.y, q.z};
+  int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
+  int width = F.width;
+  for (int k = 0; k < width; k++) {
+    int off[3]; float aij; int j;
+    if (l == 0) {
+      off[0] = off[1] = off[2] = 0;
+      if (k == 0) { aij = snap(m, 12, i); j = (int)i; }
+      else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; off[axis] = 2 * side - 1; aij = -snap(m, 6 + f6, i);
+             j = mwrap(m, 0, mk(q.x + off[0], q.y + off[1], q.z + off[2])); }
+    } else {
+      I3 o = moffset(m, l, k); off[0] = o.x; off[1] = o.y; off[2] = o.z;
+      j = mwrap(m, l, mk(q.x + o.x, q.y + o.y, q.z + o.z));
+      aij = j < 0 ? 0 : m.rows[F.row_off + (int64_t)k * F.count + i];
+    }
+    if (j < 0 || aij == 0 || !mactive(m, l, j)) continue;
+    int ju[3] = {qa[0] + off[0], qa[1] + off[1], qa[2] + off[2]};  // unwrapped neighbour
+    int base[3]; float t[3];
+    for (int a = 0; a < 3; a++) {
+      if (F.aggregate || F.f[a] == 1) { base[a] = parent_base(F, a, ju[a]); t[a] = 0; }
+      else { float pos = ((float)ju[a] + .5f) / (float)F.f[a] - .5f; base[a] = (int)floorf(pos); t[a] = fract(pos); }
+    }
+    int nb0 = F.aggregate ? 1 : F.f[0], nb1 = F.aggregate ? 1 : F.f[1], nb2 = F.aggregate ? 1 : F.f[2];
+    for (int z = 0; z < nb2; z++) for (int y = 0; y < nb1; y++) for (int x = 0; x < nb0; x++) {
+      int bit[3] = {x, y, z}; float w = 1; int P[3];
+      for (int a = 0; a < 3; a++) { P[a] = base[a] + bit[a]; if (!F.aggregate && F.f[a] > 1) w *= bit[a] == 1 ? t[a] : 1 - t[a]; }
+      int J = parent_node(m, lc, P);
+      if (J < 0 || w == 0 || !mactive(m, lc, J)) continue;
+      int b = (P[0] - anchor[0]) + E0 * ((P[1] - anchor[1]) + E1 * (P[2] - anchor[2]));
+      acc[b * 128 + tid] += aij * w;
+    }
+  }
+  float* T = m.ap + i;
+  for (int b = 0; b < E; b++) T[(int64_t)b * F.count] = acc[b * 128 + tid];
+}
+#define GAL_THREADS 64
+__global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  extern __shared__ float acc[];  // width(l+1) * GAL_THREADS, one slice per lane
   int lc = l + 1; const MGLevel& C = m.lv[lc]; const MGLevel& F = m.lv[l];
-  int64_t I = GID(); int tid = threadIdx.x; int W = C.width;
+  int G = C.lanes > 8 ? 8 : C.lanes;
+  int64_t gid = GID(); int tid = threadIdx.x;
+  int64_t I = gid / G; int lane = (int)(gid % G);
+  int W = C.width;
   for (int k = 0; k < W; k++) acc[k * GAL_THREADS + tid] = 0;
   const signed char* slots = m.slots + 125 * lc;
   bool live = I < C.count && mactive(m, lc, (int)I);
+  int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2], E = E0 * E1 * E2;
   if (live) {
     I3 qc = coord(I, mdims(m, lc)); I3 nf = mdims(m, l);
     int lo[3], hi[3]; int qca[3] = {qc.x, qc.y, qc.z}; int nfa[3] = {nf.x, nf.y, nf.z};
@@ -377,38 +553,35 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin(Ctx s, MGDev m, int
         if (!periodic) { if (lo[a] < 0) lo[a] = 0; if (hi[a] > nfa[a] - 1) hi[a] = nfa[a] - 1; }
       }
     }
-    for (int z = lo[2]; z <= hi[2]; z++) for (int y = lo[1]; y <= hi[1]; y++) for (int x = lo[0]; x <= hi[0]; x++) {
+    int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1, total = sx * sy * sz;
+    for (int c = lane; c < total; c += G) {
+      int x = lo[0] + c % sx, y = lo[1] + (c / sx) % sy, z = lo[2] + c / (sx * sy);
       int i = mwrap(m, l, mk(x, y, z)); if (i < 0 || !mactive(m, l, i)) continue;
       int ids[8]; float ws[8]; int ni = mtransfers(m, l, i, ids, ws);
       float wI = 0; for (int a = 0; a < ni; a++) if (ids[a] == (int)I) wI += ws[a];
       if (wI == 0) continue;
-      I3 q = coord(i, nf);
-      int width = l == 0 ? 7 : F.width;
-      for (int k = 0; k < width; k++) {
-        int j; float aij;
-        if (l == 0) {
-          if (k == 0) { j = i; aij = snap(m, 12, i); }
-          else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1); j = mwrap(m, 0, nb); if (j < 0) continue; aij = -snap(m, 6 + f6, i); }
-        } else {
-          I3 off = moffset(m, l, k); j = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z)); if (j < 0) continue;
-          aij = m.rows[F.row_off + (int64_t)k * F.count + i];
-        }
-        if (aij == 0 || !mactive(m, l, j)) continue;
-        int jd[8]; float jw[8]; int nj = mtransfers(m, l, j, jd, jw);
-        for (int b = 0; b < nj; b++) {
-          I3 d = mdelta(m, lc, (int)I, jd[b]);
-          int sl = in_box2(d) ? slots[slot125(d)] : -1;
-          if (sl < 0) { s.c[7] = 31; continue; }
-          acc[sl * GAL_THREADS + tid] += wI * aij * jw[b];
-        }
+      I3 q = coord(i, nf); int qa[3] = {q.x, q.y, q.z};
+      int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
+      const float* T = m.ap + i;
+      for (int b = 0; b < E; b++) {
+        float v = T[(int64_t)b * F.count];
+        if (v == 0) continue;
+        int P[3] = {anchor[0] + b % E0, anchor[1] + (b / E0) % E1, anchor[2] + b / (E0 * E1)};
+        int J = parent_node(m, lc, P);
+        I3 d = mdelta(m, lc, (int)I, J);
+        int sl = in_box2(d) ? slots[slot125(d)] : -1;
+        if (sl < 0) { s.c[7] = 31; continue; }
+        acc[sl * GAL_THREADS + tid] += wI * v;
       }
     }
   }
-  if (I >= C.count) return;
+  __syncthreads();
+  if (I >= C.count || lane != 0) return;
   float* row = m.rows + C.row_off + I;
   for (int k = 0; k < W; k++) {
-    I3 off = moffset(m, lc, k);
-    float v = live ? acc[k * GAL_THREADS + tid] : ((off.x == 0 && off.y == 0 && off.z == 0) ? 1.0f : 0.0f);
+    float v;
+    if (live) { v = 0; for (int g = 0; g < G; g++) v += acc[k * GAL_THREADS + tid + g]; }
+    else { I3 off = moffset(m, lc, k); v = (off.x == 0 && off.y == 0 && off.z == 0) ? 1.0f : 0.0f; }
     row[(int64_t)k * C.count] = v;
   }
 }
@@ -475,7 +648,9 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
   hipLaunchKernelGGL(k_mg_l1_fine, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m);
   for (int l = 0; l + 1 < m.L; l++) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
-    hipLaunchKernelGGL(k_mg_galerkin, dim3(nblocks(m.lv[l + 1].count, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
+    int G = m.lv[l + 1].lanes > 8 ? 8 : m.lv[l + 1].lanes;
+    hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), 0, st, s, m, l);
+    hipLaunchKernelGGL(k_mg_galerkin_ptap, dim3(nblocks((int64_t)m.lv[l + 1].count * G, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
     hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
     hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
   }
@@ -485,9 +660,11 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
 
 struct CycleBuilder {
   const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
+  unsigned coop_blocks(int l) { return nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256); }
   int smooth(int l, int field) {
     for (int sweep = 0; sweep < smoothing; sweep++) {
-      hipLaunchKernelGGL(k_mg_cheb, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      if (l == 0) hipLaunchKernelGGL(k_mg_cheb, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      else hipLaunchKernelGGL(k_mg_coarse<0>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
     return field;
@@ -498,14 +675,16 @@ struct CycleBuilder {
       if (m.lv[l].count <= 64) { hipLaunchKernelGGL(k_mg_coarse_solve, dim3(1), dim3(64), 0, st, s, m, l, phase); return 1; }
       int tf = 1;
       for (int sweep = 0; sweep < 32; sweep++) {
-        hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, sweep == 0 ? 1 : 0, phase);
+        if (sweep == 0) hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, 1, phase);
+        else hipLaunchKernelGGL(k_mg_coarse<2>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, tf, 0, phase);
         tf = 3 - tf;
       }
       return tf;
     }
     int field = smooth(l, 1);
-    hipLaunchKernelGGL(k_mg_residual, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
-    hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1, phase);
+    if (l == 0) hipLaunchKernelGGL(k_mg_residual, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
+    else hipLaunchKernelGGL(k_mg_coarse<1>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
+    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(coop_blocks(l + 1)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
     return smooth(l, field);

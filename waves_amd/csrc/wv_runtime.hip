@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -30,7 +31,8 @@ struct Plan {
   std::vector<int> meta;            // 16 ints per level, the reference layout
   std::vector<int> offsets;         // 3 ints per stencil entry
   std::vector<signed char> slots;   // 125 per level
-  int64_t node_count = 0, row_count = 0;
+  std::vector<int> deltas;          // 1 int per stencil entry
+  int64_t node_count = 0, row_count = 0, ap_floats = 0;
 };
 
 // water/pressure_multigrid.gd:181-188
@@ -100,12 +102,42 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     L.row_off = l > 0 ? row_count : 0;
     if (l + 1 < p.L) transfer_count += (int64_t)L.count * m[14];
     if (l > 0) row_count += (int64_t)L.count * L.width;
+    L.margin[0] = L.margin[1] = L.margin[2] = 0;
     for (int k = 0; k < L.width; k++) {
       const Offset& o = stencils[l][k];
+      L.margin[0] = std::max(L.margin[0], std::abs(o.x)); L.margin[1] = std::max(L.margin[1], std::abs(o.y)); L.margin[2] = std::max(L.margin[2], std::abs(o.z));
+      p.deltas.push_back(o.x + L.n[0] * (o.y + L.n[1] * o.z));
       p.offsets.push_back(o.x); p.offsets.push_back(o.y); p.offsets.push_back(o.z);
       if (std::abs(o.x) <= 2 && std::abs(o.y) <= 2 && std::abs(o.z) <= 2) p.slots[125 * l + (o.x + 2 + 5 * (o.y + 2 + 5 * (o.z + 2)))] = (signed char)k;
       pos += 3;
     }
+  }
+  // Galerkin stage-1 boxes: coarse span of the parents of a fine row's stencil
+  p.ap_floats = 0;
+  for (int l = 0; l + 1 < p.L; l++) {
+    MGLevel& F = p.lv[l];
+    int64_t e = 1;
+    for (int a = 0; a < 3; a++) {
+      int mx = 0, m = F.margin[a], f = F.f[a];
+      for (int q = 0; q < 2 * f + 2; q++) {
+        auto pb = [&](int j) -> int {
+          if (F.aggregate) return (int)std::floor((double)j / f);
+          if (f == 1) return j;
+          return (int)std::floor(((float)j + .5f) / (float)f - .5f);
+        };
+        int span = pb(q + m) - pb(q - m) + (F.aggregate || f == 1 ? 1 : 2);
+        mx = std::max(mx, span);
+      }
+      F.box[a] = mx; e *= mx;
+    }
+    if (e > 64) return false;  // stage-1 LDS holds 64 box entries per fine row
+    p.ap_floats = std::max(p.ap_floats, e * F.count);
+  }
+  // lanes per row: enough threads to fill the chip on small levels
+  for (int l = 0; l < p.L; l++) {
+    MGLevel& L = p.lv[l]; int g = 1;
+    while (g < 32 && (int64_t)L.count * g < (1 << 18) && g * 2 <= L.width) g *= 2;
+    L.lanes = l == 0 ? 1 : g;
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
   p.node_count = node_count; p.row_count = row_count;
@@ -220,21 +252,24 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   memset(&m, 0, sizeof m);
   m.L = s->plan.L; m.perx = c.perx > .5f;
   for (int l = 0; l < m.L; l++) m.lv[l] = s->plan.lv[l];
-  int* offs; signed char* slots;
+  int* offs; signed char* slots; int* deltas;
   ALLOC(offs, s->plan.offsets.size());
+  ALLOC(deltas, s->plan.deltas.size());
   ALLOC(slots, s->plan.slots.size());
   ALLOC(m.vec, 8 * s->plan.node_count);
   ALLOC(m.rows, s->plan.row_count);
   int last = s->plan.lv[m.L - 1].count;
   ALLOC(m.linv, (int64_t)last * last);
+  ALLOC(m.ap, s->plan.ap_floats);
   ALLOC(m.snapshot, 9 * c.N);
   ALLOC(m.previous, 9 * c.N);
   ALLOC(m.state, 64 + 32 * s->history_capacity);
 #undef ALLOC
-  m.offsets = offs; m.slots = slots;
+  m.offsets = offs; m.slots = slots; m.deltas = deltas;
   m.lag = (int)o[2]; m.mode = (int)o[4]; m.capacity = (int)s->history_capacity;
   if (hipMemcpy(offs, s->plan.offsets.data(), s->plan.offsets.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(slots, s->plan.slots.data(), s->plan.slots.size(), hipMemcpyHostToDevice) != hipSuccess) { release(s); delete s; return WV_EHIP; }
+      hipMemcpy(slots, s->plan.slots.data(), s->plan.slots.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(deltas, s->plan.deltas.data(), s->plan.deltas.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) { release(s); delete s; return WV_EHIP; }
   if (hipHostMalloc((void**)&s->quota_host, 64) != hipSuccess || hipHostMalloc((void**)&s->ctrl_host, WV_NCTRL * sizeof(float)) != hipSuccess ||
       hipEventCreateWithFlags(&s->ctrl_event, hipEventDisableTiming) != hipSuccess) { release(s); delete s; return WV_EHIP; }
   memset(s->ctrl_host, 0, WV_NCTRL * sizeof(float));
