@@ -98,8 +98,14 @@ def main():
     # per cell = reads b, x, active, 6 coefficients, l1 D^-1, d (11 x 4 B) + writes x', d (2 x 4 B)
     N = s.cells
     cheb_us = solver.time_kernel(0, reps=200)
+    copy_us = solver.time_kernel(3, reps=200)
     cheb_bytes = 52 * N
     achieved = cheb_bytes / (cheb_us * 1e-6) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_mg_cheb_level0.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     result = {
         "metric": METRIC,
         "value": value,
@@ -119,7 +125,8 @@ def main():
                    "rejections": float(c[3]) - rej_start, "failure": int(c[7])},
         "roofline": {"bound": "hbm", "kernel": "mg level-0 Chebyshev sweep", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_us": cheb_us, "bytes_per_launch": cheb_bytes},
+                     "traffic": traffic, "avg_us": cheb_us, "bytes_per_launch": cheb_bytes,
+                     "copy_GBs": 8 * N / (copy_us * 1e-6) / 1e9},
     }
     solver.close()
     if not args.no_cpu:

@@ -110,7 +110,8 @@ int wv_levels(wv_solver *s, int *meta16, int max_levels);
 /* Average device duration [us] of one launch of a hot kernel, timed with HIP
  * events on the solver's stream over `reps` launches on the current state
  * (kernel: 0 = level-0 Chebyshev sweep, 1 = fine masked 7-point SpMV,
- * 2 = level-0 residual). Does not change the solver state. */
+ * 2 = level-0 residual, 3 = plain copy of N floats, the bandwidth reference).
+ * Writes only V-cycle / PCG / Galerkin scratch: the solver state is unchanged. */
 int wv_time_kernel(wv_solver *s, int kernel, int reps, float *avg_us);
 
 #ifdef __cplusplus

@@ -718,7 +718,13 @@ __global__ void __launch_bounds__(256) k_mg_residual_timed(Ctx s, MGDev m) {
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   m.vec[at(m, 0, 6, i)] = m.vec[at(m, 0, 0, i)] - mapply(m, 0, 1, (int)i);
 }
+// plain copy of N floats (the PCG scratch A p into the Galerkin scratch): the
+// bandwidth reference for counters and the roofline
+__global__ void __launch_bounds__(256) k_copy_timed(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  int64_t i = GID(); if (i < n) dst[i] = src[i];
+}
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
+  if (kernel == 3) { hipLaunchKernelGGL(k_copy_timed, dim3(nblocks(c.N, 256)), dim3(256), 0, st, c.work + 3 * c.N, m.ap, c.N); return; }
   if (kernel == 0) hipLaunchKernelGGL(k_mg_cheb_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
   else if (kernel == 2) hipLaunchKernelGGL(k_mg_residual_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
   else wv_launch_spmv_timed(c, st);

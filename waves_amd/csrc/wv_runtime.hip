@@ -141,6 +141,7 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
   p.node_count = node_count; p.row_count = row_count;
+  p.ap_floats = std::max(p.ap_floats, (int64_t)p.lv[0].count);  // also the copy kernel's destination
   return true;
 }
 
@@ -504,7 +505,7 @@ extern "C" int wv_levels(wv_solver* s, int* meta16, int max_levels) {
 
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st);
 extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us) {
-  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 2) return WV_EINVAL;
+  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 3) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
   hipEvent_t a, b;
   HIPCHECK(s, hipEventCreate(&a)); HIPCHECK(s, hipEventCreate(&b));
