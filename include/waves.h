@@ -65,7 +65,8 @@ enum {
   WV_FIELD_SNAPSHOT = 14,     /* 9 N float, preconditioner snapshot */
   WV_FIELD_AUDIT = 15,        /* hierarchy audit state (uint32) */
   WV_FIELD_MG_ROWS = 16,      /* coarse operator rows, k-major, levels >= 1 */
-  WV_FIELD_MG_LINV = 17       /* last level: dense L^-1, row-major */
+  WV_FIELD_MG_LINV = 17,      /* last level: dense L^-1, row-major */
+  WV_FIELD_PRESSURE64 = 18    /* N double: pressure of the last standalone solve in FP64 */
 };
 
 /* _init(settings) + start() + _initialize (water/pressure_gpu.gd:40-47, 82-127):
@@ -113,6 +114,32 @@ int wv_levels(wv_solver *s, int *meta16, int max_levels);
  * 2 = level-0 residual, 3 = plain copy of N floats, the bandwidth reference).
  * Writes only V-cycle / PCG / Galerkin scratch: the solver state is unchanged. */
 int wv_time_kernel(wv_solver *s, int kernel, int reps, float *avg_us);
+
+/* Report of one standalone pressure solve. */
+typedef struct wv_solve_report {
+  int iterations;                  /* PCG iterations (c[8]) */
+  int converged;                   /* c[6] at the end */
+  int failure;                     /* c[7] (1 breakdown, 2 NaN, 3 > 4096 iterations) */
+  int restarts;                    /* mixed form: residual replacements */
+  float device_ms;                 /* HIP-event time, residual initialisation .. projection */
+  float wall_ms;                   /* host wall time of the same span */
+  double true_rel_residual;        /* |b - A p| / |b| recomputed in FP64 on the device */
+  float recurrence_rel_residual;   /* c[13] */
+} wv_solve_report;
+
+/* Standalone pressure solve on the current state (no reference counterpart:
+ * the reference only solves inside its batch program). Opens a trial, rebuilds
+ * the hierarchy when the audit asks for it, sets p = 0, forms b from the
+ * candidate face velocities (op 3, optionally shifted to zero mean over the
+ * wet cells), iterates PCG to `tolerance` with the reference's convergence
+ * predicates, and projects (op 11).  mode 0: plain FP32 MGPCG (the reference
+ * ops); mode 1: BF16 V-cycle operators, FP32 vectors, FP64 dot products and an
+ * FP64 true-residual confirmation; a converged flag the true residual does not
+ * confirm folds p into an FP64 accumulator, replaces r by the true residual
+ * and restarts CG on the FP32 correction.  The FP64 solution is left in
+ * WV_FIELD_PRESSURE64, its FP32 rounding in the candidate bank's p, and
+ * out->true_rel_residual is computed from the FP64 solution. */
+int wv_pressure_solve(wv_solver *s, int mode, float tolerance, int max_iterations, int zero_mean, wv_solve_report *out);
 
 #ifdef __cplusplus
 }

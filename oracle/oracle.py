@@ -147,7 +147,8 @@ class Oracle:
         for b in range(1, max_batches + 1):
             self.batch(float(target))
             c = self.control
-            if c[2] >= target or c[7] > 0.5:
+            f = c[7]
+            if c[2] >= target or (f > 0.5 and not ((f == 20 or 10 <= f <= 12) and c[29] < 2)):
                 return b
         return max_batches
 

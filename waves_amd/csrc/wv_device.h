@@ -42,6 +42,9 @@ struct Ctx {
   float4* img_planes;
   float* img_kinds;
   float4* partial;
+  double4* partial64;  // FP64 per-group partials (mixed-precision solve)
+  double* dctl;        // FP64 scalars of the mixed-precision solve: rho, b2, r2, true r2, mean b
+  double* pacc;        // FP64 pressure accumulator of the mixed-precision solve (N)
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
 };
 

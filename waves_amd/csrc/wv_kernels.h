@@ -28,6 +28,8 @@ struct MGDev {
   float* rows;               // coarse operator rows, k-major, FP32
   float* linv;               // last level: dense L^-1 (count^2)
   float* ap;                 // Galerkin scratch: one fine level's (A P) rows, box-major
+  uint16_t* rows16;          // BF16 copy of the coarse rows (mixed-precision V-cycle)
+  uint16_t* snap16;          // BF16 copy of the 7 level-0 snapshot coefficient fields
   float* snapshot;           // 9 N: preconditioner's fine coefficients, kind, face signature
   float* previous;           // 9 N: previous trial's operator audit copy
   unsigned* state;           // hierarchy audit header + per-trial records
@@ -43,3 +45,12 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st);
 void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st);
 void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st);
 void wv_launch_spmv_timed(const Ctx& s, hipStream_t st);
+// mixed-precision V-cycle: BF16 operators, FP32 vectors and accumulation
+void wv_mg_to_bf16(const Ctx& s, const MGDev& m, hipStream_t st);
+void wv_mg_cycle_bf16(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, bool fp64_out, hipStream_t st);
+// standalone pressure solve pieces (wv_solve.hip)
+void wv_solve_begin(const Ctx& s, bool zero_pressure, hipStream_t st);
+void wv_solve_zero_mean(const Ctx& s, hipStream_t st);
+void wv_solve_dot_alpha64(const Ctx& s, hipStream_t st);
+void wv_solve_true_residual64(const Ctx& s, const MGDev& m, bool replace, int parts, hipStream_t st);
+void wv_solve_finish(const Ctx& s, hipStream_t st);

@@ -15,81 +15,7 @@
 // accumulator in LDS), then the lower triangle is copied from the upper one so
 // the coarse operator is exactly symmetric, as the reference's canonical upper
 // triangle storage is.  The build is deterministic.
-#include "wv_kernels.h"
-
-#define GID() ((int64_t)blockIdx.x * blockDim.x + threadIdx.x)
-
-__device__ __forceinline__ int pmod(int x, int n) { return x >= 0 ? x % n : (n - 1) - ((-1 - x) % n); }
-__device__ __forceinline__ int64_t at(const MGDev& m, int l, int field, int64_t i) { return m.lv[l].vec_off + (int64_t)field * m.lv[l].count + i; }
-__device__ __forceinline__ I3 mdims(const MGDev& m, int l) { return mk(m.lv[l].n[0], m.lv[l].n[1], m.lv[l].n[2]); }
-// pressure_multigrid.glslinc:36-47
-__device__ __forceinline__ int mwrap(const MGDev& m, int l, I3 q) {
-  I3 n = mdims(m, l); q.z = pmod(q.z, n.z);
-  if (m.perx) q.x = pmod(q.x, n.x);
-  if (q.x < 0 || q.x >= n.x || q.y < 0 || q.y >= n.y) return -1;
-  return (int)index3(q, n);
-}
-__device__ __forceinline__ I3 mdelta(const MGDev& m, int l, int i, int j) {
-  I3 n = mdims(m, l), a = coord(i, n), b = coord(j, n);
-  I3 d = mk(b.x - a.x, b.y - a.y, b.z - a.z);
-  if (m.perx) d.x = pmod(d.x + n.x / 2, n.x) - n.x / 2;
-  d.z = pmod(d.z + n.z / 2, n.z) - n.z / 2;
-  return d;
-}
-__device__ __forceinline__ int slot125(I3 d) { return d.x + 2 + 5 * (d.y + 2 + 5 * (d.z + 2)); }
-__device__ __forceinline__ bool in_box2(I3 d) { return abs(d.x) <= 2 && abs(d.y) <= 2 && abs(d.z) <= 2; }
-__device__ __forceinline__ I3 moffset(const MGDev& m, int l, int k) { const int* o = m.offsets + 3 * (m.lv[l].off_start + k); return mk(o[0], o[1], o[2]); }
-__device__ __forceinline__ bool mactive(const MGDev& m, int l, int i) { return i >= 0 && m.vec[at(m, l, 3, i)] > .5f; }
-__device__ __forceinline__ float snap(const MGDev& m, int field, int64_t i) { return m.snapshot[(int64_t)(field - 6) * m.lv[0].count + i]; }
-
-// pressure_multigrid.glslinc:90-105
-__device__ __forceinline__ float mapply(const MGDev& m, int l, int field, int i) {
-  if (!mactive(m, l, i)) return m.vec[at(m, l, field, i)];
-  float sum = 0; I3 q = coord(i, mdims(m, l));
-  if (l == 0) {
-    float here = m.vec[at(m, 0, field, i)];
-#pragma unroll
-    for (int f6 = 0; f6 < 6; f6++) {
-      int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
-      int j = mwrap(m, 0, nb);
-      float other = mactive(m, 0, j) ? m.vec[at(m, 0, field, j)] : 0;
-      sum += snap(m, 6 + f6, i) * (here - other);
-    }
-    return sum;
-  }
-  int width = m.lv[l].width, N = m.lv[l].count; const float* row = m.rows + m.lv[l].row_off + i;
-  for (int k = 0; k < width; k++) {
-    I3 off = moffset(m, l, k); int j = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
-    sum += row[(int64_t)k * N] * m.vec[at(m, l, field, j >= 0 ? j : 0)];
-  }
-  return sum;
-}
-// pressure_multigrid.glslinc:108-120: transfers of fine node i (level l) to level l+1
-__device__ __forceinline__ int mtransfers(const MGDev& m, int l, int i, int* ids, float* w) {
-  const MGLevel& L = m.lv[l]; I3 n = mdims(m, l + 1), q = coord(i, mdims(m, l));
-  if (L.aggregate) {
-    int j = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), n);
-    if (!mactive(m, l + 1, j)) return 0; ids[0] = j; w[0] = 1; return 1;
-  }
-  int base[3]; float t[3]; int qq[3] = {q.x, q.y, q.z};
-  for (int k = 0; k < 3; k++) { float pos = ((float)qq[k] + .5f) / (float)L.f[k] - .5f; base[k] = (int)floorf(pos); t[k] = fract(pos); }
-  int size = 0;
-  for (int z = 0; z < L.f[2]; z++) for (int y = 0; y < L.f[1]; y++) for (int x = 0; x < L.f[0]; x++) {
-    int bit[3] = {x, y, z}; I3 nb = mk(base[0] + x, base[1] + y, base[2] + z); float ww = 1;
-    for (int axis = 0; axis < 3; axis++) if (L.f[axis] > 1) ww *= bit[axis] == 1 ? t[axis] : 1 - t[axis];
-    if (nb.y >= n.y) continue; if (nb.y < 0) nb.y = 0;
-    nb.x = m.perx ? pmod(nb.x, n.x) : iclamp(nb.x, 0, n.x - 1); nb.z = pmod(nb.z, n.z);
-    int j = (int)index3(nb, n); if (!mactive(m, l + 1, j) || ww == 0) continue;
-    ids[size] = j; w[size] = ww; size++;
-  }
-  return size;
-}
-__device__ __forceinline__ bool mg_enabled(const Ctx& s, int phase) {
-  const float* c = s.c;
-  if (c[7] > .5f) return false;
-  if (phase == 0) return (c[5] > .5f || c[57] == 2) && c[6] < .5f;
-  return c[4] > .5f && c[6] < .5f;
-}
+#include "wv_mgdev.h"
 
 // ------------------------------------------------------------ V-cycle -------
 // pressure_multigrid.glslinc:191 (op 5): b = r, x1 = x2 = 0 on level 0
@@ -99,10 +25,11 @@ __global__ void __launch_bounds__(256) k_mg_load(Ctx s, MGDev m, int phase) {
   m.vec[at(m, 0, 0, i)] = s.work[i]; m.vec[at(m, 0, 1, i)] = 0; m.vec[at(m, 0, 2, i)] = 0;
 }
 // pressure_multigrid.glslinc:192-196: 4th-kind Chebyshev sweep k, l1-Jacobi scaled
+template <bool H>
 __global__ void __launch_bounds__(256) k_mg_cheb(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
-  float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
+  float r = m.vec[at(m, l, 0, i)] - mapply0<H>(m, source, (int)i);
   float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + d;
 }
@@ -122,6 +49,7 @@ __device__ __forceinline__ float group_sum(float v, int G) {
   return v;
 }
 // sum_k A[row, k] v_field[col_k]; every lane returns the row total
+template <bool H>
 __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
   const MGLevel& L = m.lv[l]; int N = L.count, W = L.width;
   float sum = 0;
@@ -130,13 +58,14 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
     bool interior = q.x >= L.margin[0] && q.x < L.n[0] - L.margin[0] && q.y >= L.margin[1] && q.y < L.n[1] - L.margin[1] &&
                     q.z >= L.margin[2] && q.z < L.n[2] - L.margin[2];
     const float* row = m.rows + L.row_off + rc.row;
+    const uint16_t* row16 = m.rows16 + L.row_off + rc.row;
     const float* v = m.vec + L.vec_off + (int64_t)field * N;
     if (interior) {
-      for (int k = rc.lane; k < W; k += rc.G) sum += row[(int64_t)k * N] * v[rc.row + sh_delta[k]];
+      for (int k = rc.lane; k < W; k += rc.G) sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[rc.row + sh_delta[k]];
     } else {
       for (int k = rc.lane; k < W; k += rc.G) {
         int j = mwrap(m, l, mk(q.x + sh_off[3 * k], q.y + sh_off[3 * k + 1], q.z + sh_off[3 * k + 2]));
-        sum += row[(int64_t)k * N] * v[j >= 0 ? j : 0];
+        sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[j >= 0 ? j : 0];
       }
     }
   }
@@ -151,14 +80,14 @@ __device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off
   __syncthreads();
 }
 // OP 0: Chebyshev sweep k from `source`; 1: residual of `source`; 2: Jacobi sweep from `source`
-template <int OP>
+template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ int sh_off[3 * 125], sh_delta[125];
   stage_stencil(m, l, sh_off, sh_delta);
   const MGLevel& L = m.lv[l];
   RowCtx rc = row_ctx(L.lanes, L.count);
-  float ax = coarse_apply(m, l, source, rc, sh_off, sh_delta);
+  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
   if (!rc.valid || rc.lane != 0) return;
   int i = rc.row;
   float xs = m.vec[at(m, l, source, i)];
@@ -221,10 +150,11 @@ __global__ void __launch_bounds__(256) k_mg_jacobi(Ctx s, MGDev m, int l, int so
   m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + m.vec[at(m, l, 4, i)] * (m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i));
 }
 // op 7
+template <bool H>
 __global__ void __launch_bounds__(256) k_mg_residual(Ctx s, MGDev m, int l, int field, int phase) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
-  m.vec[at(m, l, 6, i)] = m.vec[at(m, l, 0, i)] - mapply(m, l, field, (int)i);
+  m.vec[at(m, l, 6, i)] = m.vec[at(m, l, 0, i)] - mapply0<H>(m, field, (int)i);
 }
 // op 8: exact P^T residual gather into level l from level l-1
 __global__ void __launch_bounds__(256) k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
@@ -658,13 +588,14 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
   hipLaunchKernelGGL(k_audit4, dim3(1), dim3(1), 0, st, s, m);
 }
 
+template <bool H>
 struct CycleBuilder {
   const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
   unsigned coop_blocks(int l) { return nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256); }
   int smooth(int l, int field) {
     for (int sweep = 0; sweep < smoothing; sweep++) {
-      if (l == 0) hipLaunchKernelGGL(k_mg_cheb, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
-      else hipLaunchKernelGGL(k_mg_coarse<0>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
     return field;
@@ -676,14 +607,14 @@ struct CycleBuilder {
       int tf = 1;
       for (int sweep = 0; sweep < 32; sweep++) {
         if (sweep == 0) hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, 1, phase);
-        else hipLaunchKernelGGL(k_mg_coarse<2>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, tf, 0, phase);
+        else hipLaunchKernelGGL((k_mg_coarse<2, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, tf, 0, phase);
         tf = 3 - tf;
       }
       return tf;
     }
     int field = smooth(l, 1);
-    if (l == 0) hipLaunchKernelGGL(k_mg_residual, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
-    else hipLaunchKernelGGL(k_mg_coarse<1>, dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
+    if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
+    else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
     hipLaunchKernelGGL(k_mg_restrict_coop, dim3(coop_blocks(l + 1)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
@@ -693,7 +624,7 @@ struct CycleBuilder {
 
 void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st) {
   if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
-  CycleBuilder b{s, m, phase, smoothing, st};
+  CycleBuilder<false> b{s, m, phase, smoothing, st};
   int final_field = b.cycle(0);
   hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
   hipLaunchKernelGGL(k_mg_out, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
@@ -710,13 +641,13 @@ void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st) {
 __global__ void __launch_bounds__(256) k_mg_cheb_timed(Ctx s, MGDev m) {
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   int l = 0, source = 1, k = 2;
-  float r = m.vec[at(m, l, 0, i)] - mapply(m, l, source, (int)i);
+  float r = m.vec[at(m, l, 0, i)] - mapply0<false>(m, source, (int)i);
   float d = (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)] + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 2, i)] = m.vec[at(m, l, 1, i)] + d;
 }
 __global__ void __launch_bounds__(256) k_mg_residual_timed(Ctx s, MGDev m) {
   int64_t i = GID(); if (i >= m.lv[0].count) return;
-  m.vec[at(m, 0, 6, i)] = m.vec[at(m, 0, 0, i)] - mapply(m, 0, 1, (int)i);
+  m.vec[at(m, 0, 6, i)] = m.vec[at(m, 0, 0, i)] - mapply0<false>(m, 1, (int)i);
 }
 // plain copy of N floats (the PCG scratch A p into the Galerkin scratch): the
 // bandwidth reference for counters and the roofline
@@ -728,4 +659,26 @@ void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
   if (kernel == 0) hipLaunchKernelGGL(k_mg_cheb_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
   else if (kernel == 2) hipLaunchKernelGGL(k_mg_residual_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
   else wv_launch_spmv_timed(c, st);
+}
+
+// ---- mixed-precision V-cycle -------------------------------------------------
+__global__ void __launch_bounds__(256) k_mg_to_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, int64_t n) {
+  int64_t i = GID(); if (i < n) dst[i] = f32_to_bf16(src[i]);
+}
+void wv_mg_to_bf16(const Ctx& s, const MGDev& m, hipStream_t st) {
+  int64_t nr = 0; for (int l = 1; l < m.L; l++) nr += (int64_t)m.lv[l].count * m.lv[l].width;
+  if (nr) hipLaunchKernelGGL(k_mg_to_bf16, dim3(nblocks(nr, 256)), dim3(256), 0, st, (const float*)m.rows, m.rows16, nr);
+  int64_t ns = 7 * (int64_t)m.lv[0].count;
+  hipLaunchKernelGGL(k_mg_to_bf16, dim3(nblocks(ns, 256)), dim3(256), 0, st, (const float*)m.snapshot, m.snap16, ns);
+}
+void wv_launch_mg_out64(const Ctx& s, const MGDev& m, int field, int phase, hipStream_t st);
+void wv_mg_cycle_bf16(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, bool fp64_out, hipStream_t st) {
+  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
+  CycleBuilder<true> b{s, m, phase, smoothing, st};
+  int final_field = b.cycle(0);
+  if (fp64_out) wv_launch_mg_out64(s, m, final_field, phase, st);
+  else {
+    hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
+    hipLaunchKernelGGL(k_mg_out, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
+  }
 }

@@ -10,6 +10,7 @@
 // host never decides acceptance, retries or PCG convergence.
 #include <hip/hip_runtime.h>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
@@ -247,6 +248,9 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.img_planes, c.N);
   ALLOC(c.img_kinds, c.N);
   ALLOC(c.partial, (c.stride + WV_RED - 1) / WV_RED + 1);
+  ALLOC(c.partial64, (c.N + WV_RED - 1) / WV_RED + 1);
+  ALLOC(c.dctl, 16);
+  ALLOC(c.pacc, c.N);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
   MGDev& m = s->mg;
@@ -262,6 +266,8 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   int last = s->plan.lv[m.L - 1].count;
   ALLOC(m.linv, (int64_t)last * last);
   ALLOC(m.ap, s->plan.ap_floats);
+  ALLOC(m.rows16, s->plan.row_count);
+  ALLOC(m.snap16, 7 * c.N);
   ALLOC(m.snapshot, 9 * c.N);
   ALLOC(m.previous, 9 * c.N);
   ALLOC(m.state, 64 + 32 * s->history_capacity);
@@ -444,6 +450,7 @@ static void* field_ptr(wv_solver* s, int field, int64_t* bytes) {
   case WV_FIELD_AUDIT: *bytes = (64 + 32 * s->history_capacity) * 4; return m.state;
   case WV_FIELD_MG_ROWS: *bytes = s->plan.row_count * 4; return m.rows;
   case WV_FIELD_MG_LINV: { int64_t n = s->plan.lv[s->plan.L - 1].count; *bytes = n * n * 4; return m.linv; }
+  case WV_FIELD_PRESSURE64: *bytes = c.N * 8; return c.pacc;
   }
   *bytes = -1;
   return nullptr;
@@ -517,5 +524,86 @@ extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us)
   float ms = 0; HIPCHECK(s, hipEventElapsedTime(&ms, a, b));
   hipEventDestroy(a); hipEventDestroy(b);
   *avg_us = ms * 1000.0f / reps;
+  return WV_OK;
+}
+
+// Standalone pressure solve on the current geometry (see include/waves.h).
+extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int max_iterations, int zero_mean, wv_solve_report* out) {
+  if (!s || !out || (mode != 0 && mode != 1) || !(tolerance > 0) || max_iterations < 1) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  Ctx c = s->ctx; c.tol = tolerance;
+  const MGDev& m = s->mg; hipStream_t st = s->stream;
+  hipEvent_t e0, e1;
+  HIPCHECK(s, hipEventCreate(&e0)); HIPCHECK(s, hipEventCreate(&e1));
+  HIPCHECK(s, hipStreamSynchronize(st));
+  auto wall0 = std::chrono::steady_clock::now();
+  HIPCHECK(s, hipEventRecord(e0, st));
+  wv_solve_begin(c, true, st);
+  wv_mg_build(c, m, st);                       // hierarchy audit + Galerkin (skipped if unchanged)
+  if (mode == 1) wv_mg_to_bf16(c, m, st);
+  wv_launch_op(c, 3, st);                      // b, r = b - A p, z, p-dir
+  if (zero_mean) wv_solve_zero_mean(c, st);
+  auto cycle = [&](int phase, bool load) {
+    if (mode == 1) wv_mg_cycle_bf16(c, m, phase, s->smoothing, load, true, st);
+    else wv_mg_cycle(c, m, phase, s->smoothing, load, st);
+  };
+  cycle(0, true);                              // z = M r, rho, convergence (op 4)
+  const MGLevel& L0 = m.lv[0];
+  float* v = m.vec + L0.vec_off;
+  int restarts = 0, chunk = 4;
+  const char* trace = getenv("WAVES_SOLVE_TRACE");
+  if (trace && trace[0] == '1') chunk = 1;
+  float ctl[WV_NCTRL];
+  for (;;) {
+    for (int k = 0; k < chunk; k++) {
+      wv_launch_op(c, 5, st);                  // A p
+      if (mode == 1) wv_solve_dot_alpha64(c, st); else wv_launch_op(c, 6, st);
+      wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, st);
+      cycle(1, false);                         // z = M r, beta, convergence (op 9)
+      wv_launch_op(c, 10, st);                 // p = z + beta p
+    }
+    HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, c.c, sizeof ctl, hipMemcpyDeviceToHost, st));
+    HIPCHECK(s, hipStreamSynchronize(st));
+    memcpy(ctl, s->ctrl_host, sizeof ctl);
+    if (chunk == 1) {
+      double d[8];
+      HIPCHECK(s, hipMemcpyAsync(d, c.dctl, sizeof d, hipMemcpyDeviceToHost, st));
+      HIPCHECK(s, hipStreamSynchronize(st));
+      fprintf(stderr, "[solve mode %d] it %g conv %g fail %g relres %g alpha %g beta %g rho %g b2 %g r2 %g dt %g tol %g\n",
+              mode, ctl[8], ctl[6], ctl[7], ctl[13], ctl[11], ctl[12], d[0], d[1], d[2], ctl[20], c.tol);
+    }
+    if (ctl[7] > .5f || ctl[8] >= max_iterations) break;
+    if (ctl[6] > .5f) {
+      if (mode == 0) break;
+      wv_solve_true_residual64(c, m, true, 3, st);  // confirm in FP64, else fold p, replace r, restart
+      double flag = 0;
+      HIPCHECK(s, hipMemcpyAsync(&flag, c.dctl + 6, sizeof flag, hipMemcpyDeviceToHost, st));
+      HIPCHECK(s, hipStreamSynchronize(st));
+      if (flag < .5) break;
+      if (++restarts > 16) break;
+      cycle(0, true);
+    }
+  }
+  wv_solve_finish(c, st);                      // FP64 solution; candidate p = its FP32 rounding
+  wv_launch_op(c, 11, st);                     // projection (pressure_pass.glslinc:52-62)
+  HIPCHECK(s, hipEventRecord(e1, st));
+  HIPCHECK(s, hipEventSynchronize(e1));
+  auto wall1 = std::chrono::steady_clock::now();
+  float ms = 0; HIPCHECK(s, hipEventElapsedTime(&ms, e0, e1));
+  wv_solve_true_residual64(c, m, false, 2, st);   // report on the FP64 solution
+  double d[16];
+  HIPCHECK(s, hipMemcpyAsync(d, c.dctl, sizeof d, hipMemcpyDeviceToHost, st));
+  HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, c.c, sizeof ctl, hipMemcpyDeviceToHost, st));
+  HIPCHECK(s, hipStreamSynchronize(st));
+  memcpy(ctl, s->ctrl_host, sizeof ctl);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  out->iterations = (int)ctl[8];
+  out->converged = ctl[6] > .5f;
+  out->failure = (int)ctl[7];
+  out->restarts = restarts;
+  out->device_ms = ms;
+  out->wall_ms = (float)std::chrono::duration<double, std::milli>(wall1 - wall0).count();
+  out->true_rel_residual = d[4] > 0 ? std::sqrt(d[3] / d[4]) : 0;
+  out->recurrence_rel_residual = ctl[13];
   return WV_OK;
 }

@@ -25,20 +25,39 @@ FIELD_BANK_A, FIELD_BANK_B, FIELD_WORK, FIELD_GEO, FIELD_CONTROL = 0, 1, 2, 3, 4
 FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
 FIELD_HISTORY, FIELD_COASTAL_HISTORY, FIELD_PROBE, FIELD_PARTIAL = 9, 10, 11, 12
 FIELD_MG_VECTORS, FIELD_SNAPSHOT, FIELD_AUDIT, FIELD_MG_ROWS, FIELD_MG_LINV = 13, 14, 15, 16, 17
+FIELD_PRESSURE64 = 18
 
 #: every entry point declared in include/waves.h
 EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_request_time",
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
-           "wv_levels", "wv_time_kernel")
+           "wv_levels", "wv_time_kernel", "wv_pressure_solve")
 
 
 class WavesError(RuntimeError):
     pass
 
 
+def stopped(c: np.ndarray) -> bool:
+    """True when the device has stopped on a failure: a non-zero c[7] that is
+    not one of the recoverable codes still within its retry budget (CFL 20,
+    VOF bounds 10-12, at most 2 retries; coupled_motion.glslinc:351-354,
+    water/coupled_gpu.gd:61-62)."""
+    f = c[7]
+    if f < 0.5:
+        return False
+    recoverable = (f == 20 or 10 <= f <= 12) and c[29] < 2
+    return not recoverable
+
+
 class _Settings(ctypes.Structure):
     _fields_ = [("pc", ctypes.c_float * 32), ("opts", ctypes.c_float * 8)]
+
+
+class _SolveReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("failure", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("device_ms", ctypes.c_float), ("wall_ms", ctypes.c_float),
+                ("true_rel_residual", ctypes.c_double), ("recurrence_rel_residual", ctypes.c_float)]
 
 
 _lib = None
@@ -74,6 +93,7 @@ def library() -> ctypes.CDLL:
         L.wv_last_error.argtypes = [vp]
         L.wv_levels.argtypes = [vp, ctypes.POINTER(i32), i32]
         L.wv_time_kernel.argtypes = [vp, i32, i32, fp]
+        L.wv_pressure_solve.argtypes = [vp, i32, ctypes.c_float, i32, i32, ctypes.POINTER(_SolveReport)]
         _lib = L
         return L
 
@@ -214,6 +234,15 @@ class Solver:
         self._check(self._lib.wv_time_kernel(self._h, kernel, reps, ctypes.byref(out)))
         return float(out.value)
 
+    def pressure_solve(self, mode: int = 0, tolerance: float = 1e-6, max_iterations: int = 4096,
+                       zero_mean: bool = True) -> Dict:
+        """Standalone pressure solve on the current state (wv_pressure_solve):
+        mode 0 plain FP32 MGPCG, mode 1 BF16 V-cycle / FP64 dots and true residual."""
+        r = _SolveReport()
+        self._check(self._lib.wv_pressure_solve(self._h, int(mode), float(tolerance), int(max_iterations),
+                                                int(zero_mean), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _SolveReport._fields_}
+
     def step(self, max_batches: int = 64) -> int:
         """Advance until one more step is accepted (or the run stops); one
         control readback per batch. Returns the batches used."""
@@ -222,6 +251,6 @@ class Solver:
         for b in range(1, max_batches + 1):
             self.frame(1)
             c = self.poll(wait=True)
-            if c[2] >= target or c[7] > 0.5:
+            if c[2] >= target or stopped(c):
                 return b
         return max_batches
