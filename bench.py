@@ -11,7 +11,10 @@ accepted steps are timed, bracketed by device synchronisation.  Batches are
 submitted ahead with one control readback per batch (water/work_scheduler.gd),
 the quota stops the device after step K.
 
-Also reported: the roofline of the dominant kernel (level-0 Chebyshev sweep
+Also reported: the standalone pressure solve at 310x48x80 (seeded wavy
+surface, N(0, 0.1 m/s) faces, zero-mean RHS) in milliseconds per solve at
+relative residual 1e-6 and 1e-8, plain FP32 MGPCG against the mixed form
+(BF16 V-cycle, FP64 dots and true residual), the roofline of the dominant kernel (level-0 Chebyshev sweep
 of the multigrid preconditioner, timed live with HIP events on the solver's
 stream) and the CPU oracle timed on the host as a baseline.
 """
@@ -28,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from waves_amd.settings import benchmark_beach  # noqa: E402
+from waves_amd.settings import benchmark_beach, default_beach  # noqa: E402
 from waves_amd.solver import Solver, stopped  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -69,6 +72,32 @@ def cpu_baseline(sample_steps: int = 1):
             "sample": f"oracle, {sample_steps} accepted step(s) from reset at 310x48x80 ({float(c[1]):.4f} sim-s in {wall:.1f} s)"}
 
 
+def pressure_leg(repeats: int = 3):
+    """Standalone solves on the pressure case at 310x48x80: per tolerance and
+    form, one untimed solve then `repeats` solves from a fresh load; the
+    device time (HIP events, hierarchy audit .. projection) is averaged."""
+    from waves_amd.scenarios import load_pressure_case
+    s = default_beach(pressure_tolerance=1e-6)
+    g = Solver(s)
+    g.start()
+    out = {"workload": "standalone pressure solve, 310x48x80, seeded wavy surface, N(0,0.1 m/s) faces, zero-mean RHS"}
+    for tol in (1e-6, 1e-8):
+        for mode, name in ((0, "plain_fp32"), (1, "mixed_bf16_fp64")):
+            load_pressure_case(g, s)
+            g.pressure_solve(mode, tol)
+            runs = []
+            for _ in range(repeats):
+                load_pressure_case(g, s)
+                runs.append(g.pressure_solve(mode, tol))
+            ms = float(np.mean([r["device_ms"] for r in runs]))
+            r = runs[-1]
+            out[f"{name}@{tol:g}"] = {"ms": ms, "iterations": r["iterations"], "restarts": r["restarts"],
+                                      "converged": r["converged"], "true_rel_residual": r["true_rel_residual"]}
+        out[f"speedup@{tol:g}"] = out[f"plain_fp32@{tol:g}"]["ms"] / out[f"mixed_bf16_fp64@{tol:g}"]["ms"]
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +105,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-pressure", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
@@ -131,6 +161,8 @@ def main():
                      "copy_GBs": 8 * N / (copy_us * 1e-6) / 1e9},
     }
     solver.close()
+    if not args.no_pressure:
+        result["pressure"] = pressure_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(result))

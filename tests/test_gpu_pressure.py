@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from waves_amd.settings import Settings
-from waves_amd.scenarios import pressure_case
+from waves_amd.scenarios import load_pressure_case
 from waves_amd import solver as S
 from oracle import pressure64
 
@@ -14,15 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def load_case(s, g):
-    """Write the seeded input into both banks and rebuild geometry and
-    coefficients (ops 17, 18, 16).  A solve projects the candidate faces, so
-    every solve starts from a fresh load."""
-    bank = pressure_case(s)
-    g.write(S.FIELD_BANK_A, bank)
-    g.write(S.FIELD_BANK_B, bank)
-    for op in (17, 18, 16):
-        g.op(op)
-    g.synchronize()
+    load_pressure_case(g, s)
 
 
 def case_solver(n=(96, 48, 16)):

@@ -78,3 +78,16 @@ def pressure_case(s: Settings, seed: int = 3, amplitude: float = 0.08, sigma: fl
     W[Wm] = 0
     W[-1] = W[0]
     return np.concatenate([F.ravel(), U.ravel(), V.ravel(), W.ravel(), np.zeros(nx * ny * nz, np.float32)])
+
+
+def load_pressure_case(solver, s: Settings, seed: int = 3):
+    """Write pressure_case() into both banks of a started solver and derive
+    geometry, cell kinds and coefficients (ops 17, 18, 16).  A solve projects
+    the candidate faces, so every standalone solve starts from a fresh load."""
+    from . import solver as S
+    bank = pressure_case(s, seed)
+    solver.write(S.FIELD_BANK_A, bank)
+    solver.write(S.FIELD_BANK_B, bank)
+    for op in (17, 18, 16):
+        solver.op(op)
+    solver.synchronize()
