@@ -138,8 +138,16 @@ typedef struct wv_solve_report {
  * confirm folds p into an FP64 accumulator, replaces r by the true residual
  * and restarts CG on the FP32 correction.  The FP64 solution is left in
  * WV_FIELD_PRESSURE64, its FP32 rounding in the candidate bank's p, and
- * out->true_rel_residual is computed from the FP64 solution. */
-int wv_pressure_solve(wv_solver *s, int mode, float tolerance, int max_iterations, int zero_mean, wv_solve_report *out);
+ * out->true_rel_residual is computed from the FP64 solution.  mode 2: mode 1
+ * with the learned warm start of the attached network (waves_net.h): the
+ * network's two output fields span a correction of the starting pressure
+ * chosen by an FP64 Galerkin projection (it cannot increase the A-norm
+ * error), and CG uses the flexible (Polak-Ribiere) coefficient. */
+enum {
+  WV_SOLVE_ZERO_MEAN = 1,  /* shift b to zero mean over the wet cells (starts from p = 0) */
+  WV_SOLVE_GIVEN_RHS = 2   /* b is work field 5 as written by the caller; start from the candidate p */
+};
+int wv_pressure_solve(wv_solver *s, int mode, float tolerance, int max_iterations, int flags, wv_solve_report *out);
 
 #ifdef __cplusplus
 }

@@ -14,9 +14,9 @@ from waves_amd.settings import Settings, default_beach, dispersion_k
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "waves.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*\*?\s*(wv_\w+)\s*\(", text, re.M)))
+def declared_symbols(header="waves.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^\s*(?:void|int|int64_t|const char \*)\s*\*?\s*(wv_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,6 +26,15 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(S.EXPORTS)
+    net_names = declared_symbols("waves_net.h")
+    for n in net_names:
+        assert hasattr(lib, n), n
+    assert set(net_names) == set(S.NET_EXPORTS)
+
+
+def test_net_parameter_layout_matches_oracle():
+    from oracle import unet
+    assert S.library().wv_net_param_count() == unet.param_count()
 
 
 def test_create_without_device_fails_cleanly():

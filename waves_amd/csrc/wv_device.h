@@ -45,6 +45,8 @@ struct Ctx {
   double4* partial64;  // FP64 per-group partials (mixed-precision solve)
   double* dctl;        // FP64 scalars of the mixed-precision solve: rho, b2, r2, true r2, mean b
   double* pacc;        // FP64 pressure accumulator of the mixed-precision solve (N)
+  float* aux;          // 2 N scratch of the learned warm start (A v1, A v2)
+  int flex;            // flexible (Polak-Ribiere) CG coefficient in the FP64 V-cycle output
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
 };
 
@@ -123,6 +125,21 @@ __device__ __forceinline__ bool solid(const Ctx& s, I3 q) {
 __device__ __forceinline__ bool wet(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && s.geo[index3(q, dims(s))].y > 0.5f; }
 __device__ __forceinline__ float phi(const Ctx& s, I3 q) { q = wrap_cell(s, q); return outside(s, q) ? center(s, q).y - eta(s, q) : s.geo[index3(q, dims(s))].x; }
 __device__ __forceinline__ float press(const Ctx& s, I3 q) { q = wrap_cell(s, q); return wet(s, q) ? getv(s, candidate(s), P_(s) + index3(q, dims(s))) : 0.0f; }
+__device__ __forceinline__ float vector_at(const Ctx& s, I3 q, int which) {
+  q = wrap_cell(s, q); if (!wet(s, q)) return 0.0f;
+  int64_t i = index3(q, dims(s));
+  return which == 0 ? press(s, q) : s.work[2 * s.N + i];
+}
+// pressure_core.glslinc:120-125: masked 7-point operator
+__device__ __forceinline__ float operator_at(const Ctx& s, I3 q, int which) {
+  float result = 0, here = vector_at(s, q, which); int64_t i = index3(q, dims(s));
+  int face = 0;
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    I3 nb = q; addc(nb, axis, sign); float k = s.work[(int64_t)(6 + face) * s.N + i];
+    result += k * (here - vector_at(s, nb, which)); face++;
+  }
+  return result;
+}
 // pressure_core.glslinc:100-109: ghost-fluid face distance
 __device__ __forceinline__ float distance_face(const Ctx& s, I3 l, I3 r, int axis) {
   float h = s.h[axis];

@@ -508,21 +508,6 @@ __global__ void __launch_bounds__(256) k_defect_restore(Ctx s) {
 }
 
 // --------------------------------------------------------------- PCG ops ----
-__device__ __forceinline__ float vector_at(const Ctx& s, I3 q, int which) {
-  q = wrap_cell(s, q); if (!wet(s, q)) return 0.0f;
-  int64_t i = index3(q, dims(s));
-  return which == 0 ? press(s, q) : s.work[2 * s.N + i];
-}
-// pressure_core.glslinc:120-125: masked 7-point operator
-__device__ __forceinline__ float operator_at(const Ctx& s, I3 q, int which) {
-  float result = 0, here = vector_at(s, q, which); int64_t i = index3(q, dims(s));
-  int face = 0;
-  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
-    I3 nb = q; addc(nb, axis, sign); float k = s.work[(int64_t)(6 + face) * s.N + i];
-    result += k * (here - vector_at(s, nb, which)); face++;
-  }
-  return result;
-}
 // pressure_iterations.glslinc:4-13 (op 3).  The block partials of op 3 are
 // replaced by the V-cycle's output partials before op 4 reads them
 // (pressure_multigrid.glslinc:230-234), so this kernel publishes none.

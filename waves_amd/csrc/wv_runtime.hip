@@ -21,6 +21,8 @@
 #include <vector>
 #include "wv_kernels.h"
 #include "../../include/waves.h"
+#include "../../include/waves_net.h"
+#include "wv_netdev.h"
 
 namespace {
 
@@ -158,6 +160,7 @@ struct wv_solver {
   float* quota_dev = nullptr;
   float* quota_host = nullptr;   // pinned
   float* ctrl_host = nullptr;    // pinned, readback slots
+  wv_net* net = nullptr;         // learned warm start (not owned)
   hipEvent_t ctrl_event = nullptr;
   std::atomic<double> quota{0.0};
   float last_uploaded_quota = NAN;
@@ -248,9 +251,10 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.img_planes, c.N);
   ALLOC(c.img_kinds, c.N);
   ALLOC(c.partial, (c.stride + WV_RED - 1) / WV_RED + 1);
-  ALLOC(c.partial64, (c.N + WV_RED - 1) / WV_RED + 1);
+  ALLOC(c.partial64, 2 * ((c.N + WV_RED - 1) / WV_RED + 1));
   ALLOC(c.dctl, 16);
   ALLOC(c.pacc, c.N);
+  ALLOC(c.aux, 2 * c.N);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
   MGDev& m = s->mg;
@@ -528,23 +532,29 @@ extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us)
 }
 
 // Standalone pressure solve on the current geometry (see include/waves.h).
-extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int max_iterations, int zero_mean, wv_solve_report* out) {
-  if (!s || !out || (mode != 0 && mode != 1) || !(tolerance > 0) || max_iterations < 1) return WV_EINVAL;
+extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int max_iterations, int flags, wv_solve_report* out) {
+  if (!s || !out || mode < 0 || mode > 2 || !(tolerance > 0) || max_iterations < 1) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
-  Ctx c = s->ctx; c.tol = tolerance;
+  if (mode == 2 && !s->net) return fail(s, WV_EINVAL, "mode 2 needs a network (wv_solver_attach_net)");
+  const bool zero_mean = (flags & WV_SOLVE_ZERO_MEAN) != 0, given = (flags & WV_SOLVE_GIVEN_RHS) != 0;
+  Ctx c = s->ctx; c.tol = tolerance; c.flex = mode == 2;
   const MGDev& m = s->mg; hipStream_t st = s->stream;
   hipEvent_t e0, e1;
   HIPCHECK(s, hipEventCreate(&e0)); HIPCHECK(s, hipEventCreate(&e1));
   HIPCHECK(s, hipStreamSynchronize(st));
   auto wall0 = std::chrono::steady_clock::now();
   HIPCHECK(s, hipEventRecord(e0, st));
-  wv_solve_begin(c, true, st);
+  wv_solve_begin(c, !given, st);
   wv_mg_build(c, m, st);                       // hierarchy audit + Galerkin (skipped if unchanged)
-  if (mode == 1) wv_mg_to_bf16(c, m, st);
-  wv_launch_op(c, 3, st);                      // b, r = b - A p, z, p-dir
-  if (zero_mean) wv_solve_zero_mean(c, st);
+  if (mode >= 1) wv_mg_to_bf16(c, m, st);
+  if (given) wv_solve_init_given(c, st);       // r = b - A p from the caller's b and p
+  else {
+    wv_launch_op(c, 3, st);                    // b, r = b - A p, z, p-dir
+    if (zero_mean) wv_solve_zero_mean(c, st);
+  }
+  if (mode == 2) wv_solve_learned_start(c, s->net, st);
   auto cycle = [&](int phase, bool load) {
-    if (mode == 1) wv_mg_cycle_bf16(c, m, phase, s->smoothing, load, true, st);
+    if (mode >= 1) wv_mg_cycle_bf16(c, m, phase, s->smoothing, load, true, st);
     else wv_mg_cycle(c, m, phase, s->smoothing, load, st);
   };
   cycle(0, true);                              // z = M r, rho, convergence (op 4)
@@ -557,7 +567,7 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
   for (;;) {
     for (int k = 0; k < chunk; k++) {
       wv_launch_op(c, 5, st);                  // A p
-      if (mode == 1) wv_solve_dot_alpha64(c, st); else wv_launch_op(c, 6, st);
+      if (mode >= 1) wv_solve_dot_alpha64(c, st); else wv_launch_op(c, 6, st);
       wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, st);
       cycle(1, false);                         // z = M r, beta, convergence (op 9)
       wv_launch_op(c, 10, st);                 // p = z + beta p
@@ -605,5 +615,17 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
   out->wall_ms = (float)std::chrono::duration<double, std::milli>(wall1 - wall0).count();
   out->true_rel_residual = d[4] > 0 ? std::sqrt(d[3] / d[4]) : 0;
   out->recurrence_rel_residual = ctl[13];
+  return WV_OK;
+}
+
+extern "C" int wv_solver_attach_net(wv_solver* s, wv_net* net) {
+  if (!s) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  if (net) {
+    int d[4]; wv_net_grid(net, d);
+    if (d[0] != s->ctx.nx || d[1] != s->ctx.ny || d[2] != s->ctx.nz || d[3] != 1)
+      return fail(s, WV_EINVAL, "network grid differs from the solver grid or batch != 1");
+  }
+  s->net = net;
   return WV_OK;
 }

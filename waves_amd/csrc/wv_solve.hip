@@ -12,6 +12,7 @@
 // iterative refinement), so the FP64 solution reaches tolerances below FP32's
 // rounding floor.  No reference counterpart beyond the plain form.
 #include "wv_mgdev.h"
+#include "wv_netdev.h"
 
 static inline unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -111,9 +112,11 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out64(Ctx s, MGDev m, int field, 
   if (!final) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
     if (i < N) {
-      float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i];
+      float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i], zold = s.work[N + i];
       s.work[N + i] = z; if (phase == 0) s.work[2 * N + i] = z;
-      v = mkd((double)r * z, (double)r * r, (double)b * b, fabs((double)r));
+      // phase 1 keeps |b|^2 from phase 0; the flexible form needs r . z_old instead
+      double third = (phase == 1 && s.flex) ? (double)r * zold : (double)b * b;
+      v = mkd((double)r * z, (double)r * r, third, fabs((double)r));
     }
     v = group_reduce_d(v); put_partial(s, v); return;
   }
@@ -127,7 +130,8 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out64(Ctx s, MGDev m, int field, 
     if (c[58] == 0) c[59] = (float)v.z;
     c[57] = 0;
   } else {
-    double beta = v.x / (d[0] > 1e-30 ? d[0] : 1e-30);
+    // Fletcher-Reeves form, or Polak-Ribiere (r.z - r.z_old) / rho for the flexible solve
+    double beta = (s.flex ? v.x - v.z : v.x) / (d[0] > 1e-30 ? d[0] : 1e-30);
     d[0] = v.x; d[2] = v.y;
     c[12] = (float)beta; c[9] = (float)v.x; c[8]++; c[18]++;
     double rel = sqrt(v.y / (d[1] > 1e-30 ? d[1] : 1e-30));
@@ -210,4 +214,115 @@ __global__ void __launch_bounds__(256) k_solve_finish(Ctx s) {
 }
 void wv_solve_finish(const Ctx& s, hipStream_t st) {
   hipLaunchKernelGGL(k_solve_finish, dim3(nb(s.N, 256)), dim3(256), 0, st, s);
+}
+
+// Given right-hand side: b already in work[5N] (written by the caller), r = b -
+// A p from the candidate pressure (the operator of pressure_core.glslinc:120-125),
+// z = r / diag, direction z (pressure_iterations.glslinc:4-13 without the
+// divergence).
+__global__ void __launch_bounds__(256) k_init_given(Ctx s) {
+  int64_t i = GID(); if (i >= s.N) return;
+  int64_t N = s.N; I3 q = coord(i, dims(s));
+  float r = 0, z = 0, diag = 1;
+  if (wet(s, q)) { diag = s.work[12 * N + i]; r = s.work[5 * N + i] - operator_at(s, q, 0); z = r / diag; }
+  s.work[i] = r; s.work[N + i] = z; s.work[2 * N + i] = z; s.work[4 * N + i] = diag;
+}
+void wv_solve_init_given(const Ctx& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_given, dim3(nb(s.N, 256)), dim3(256), 0, st, s);
+}
+
+// ------------------------------------------------- learned warm start ----
+// Network input per cell: F, r / rms_wet(r), p / rms_wet(p), wet, 0 x 4 (bf16).
+__global__ void __launch_bounds__(WV_RED) k_net_scale(Ctx s, int final) {
+  int64_t N = s.N;
+  if (!final) {
+    int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
+    if (i < N && wet(s, coord(i, dims(s)))) {
+      double r = s.work[i], p = getv(s, candidate(s), P_(s) + i);
+      v = mkd(r * r, p * p, 1, 0);
+    }
+    v = group_reduce_d(v); put_partial(s, v); return;
+  }
+  d4 g = global_reduce_d(s.partial64, (N + WV_RED - 1) / WV_RED);
+  if (threadIdx.x == 0) {
+    double n = g.z > 0 ? g.z : 1;
+    s.dctl[9] = sqrt(g.x / n); s.dctl[10] = sqrt(g.y / n);
+  }
+}
+__global__ void __launch_bounds__(256) k_net_features(Ctx s, uint16_t* x) {
+  int64_t i = GID(); if (i >= s.N) return;
+  int cb = candidate(s);
+  bool w = wet(s, coord(i, dims(s)));
+  double sr = s.dctl[9], sp = s.dctl[10];
+  float f0 = getv(s, cb, i);
+  float f1 = (w && sr > 0) ? (float)((double)s.work[i] / sr) : 0.f;
+  float f2 = (w && sp > 0) ? (float)((double)getv(s, cb, P_(s) + i) / sp) : 0.f;
+  uint16_t* o = x + i * 8;
+  o[0] = f32_to_bf16(f0); o[1] = f32_to_bf16(f1); o[2] = f32_to_bf16(f2); o[3] = f32_to_bf16(w ? 1.f : 0.f);
+  o[4] = 0; o[5] = 0; o[6] = 0; o[7] = 0;
+}
+// basis vectors: v1 = fine head, v2 = coarse head prolonged by 4 per axis, both
+// zero outside the wet cells
+__device__ __forceinline__ float ws_v(const Ctx& s, const float* qf, const float* qc, int which, I3 q) {
+  q = wrap_cell(s, q);
+  if (!wet(s, q)) return 0.f;
+  I3 d = dims(s);
+  if (which == 0) return qf[index3(q, d)];
+  int n2x = ((d.x + 1) / 2 + 1) / 2, n2y = ((d.y + 1) / 2 + 1) / 2;
+  return qc[(q.x >> 2) + (int64_t)n2x * ((q.y >> 2) + (int64_t)n2y * (q.z >> 2))];
+}
+__device__ __forceinline__ float ws_apply(const Ctx& s, const float* qf, const float* qc, int which, I3 q, int64_t i) {
+  float here = ws_v(s, qf, qc, which, q), result = 0; int face = 0;
+  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
+    I3 nb = q; addc(nb, axis, sign);
+    result += s.work[(int64_t)(6 + face) * s.N + i] * (here - ws_v(s, qf, qc, which, nb)); face++;
+  }
+  return result;
+}
+// Galerkin correction on span{v1, v2}: minimise the A-norm error of p + a v1 + c v2,
+// [v1 A v1, v1 A v2; v2 A v1, v2 A v2] [a; c] = [v1 . r; v2 . r], in FP64
+__global__ void __launch_bounds__(WV_RED) k_ws_dots(Ctx s, const float* qf, const float* qc, int final) {
+  int64_t N = s.N, groups = (N + WV_RED - 1) / WV_RED;
+  if (!final) {
+    int64_t i = GID(); d4 v = mkd(0, 0, 0, 0), u = mkd(0, 0, 0, 0);
+    if (i < N) {
+      I3 q = coord(i, dims(s));
+      if (wet(s, q)) {
+        double v1 = ws_v(s, qf, qc, 0, q), v2 = ws_v(s, qf, qc, 1, q), r = s.work[i];
+        double a1 = ws_apply(s, qf, qc, 0, q, i), a2 = ws_apply(s, qf, qc, 1, q, i);
+        v = mkd(v1 * a1, v1 * a2, v2 * a2, 0); u = mkd(v1 * r, v2 * r, 0, 0);
+      }
+    }
+    v = group_reduce_d(v); u = group_reduce_d(u);
+    if (threadIdx.x == 0) { s.partial64[blockIdx.x] = make_double4(v.x, v.y, v.z, v.w); s.partial64[groups + 1 + blockIdx.x] = make_double4(u.x, u.y, u.z, u.w); }
+    return;
+  }
+  d4 a = global_reduce_d(s.partial64, groups);
+  d4 b = global_reduce_d(s.partial64 + groups + 1, groups);
+  if (threadIdx.x != 0) return;
+  double A11 = a.x, A12 = a.y, A22 = a.z, b1 = b.x, b2 = b.y, al = 0, ce = 0;
+  double det = A11 * A22 - A12 * A12;
+  if (A11 > 0 && A22 > 0 && det > 1e-10 * A11 * A22) { al = (b1 * A22 - b2 * A12) / det; ce = (A11 * b2 - A12 * b1) / det; }
+  else if (A11 > 0) al = b1 / A11;
+  else if (A22 > 0) ce = b2 / A22;
+  if (!isfinite(al) || !isfinite(ce)) { al = 0; ce = 0; }
+  s.dctl[11] = al; s.dctl[12] = ce;
+}
+__global__ void __launch_bounds__(256) k_ws_update(Ctx s, const float* qf, const float* qc) {
+  int64_t i = GID(); if (i >= s.N) return;
+  I3 q = coord(i, dims(s)); if (!wet(s, q)) return;
+  double v = s.dctl[11] * ws_v(s, qf, qc, 0, q) + s.dctl[12] * ws_v(s, qf, qc, 1, q);
+  int cb = candidate(s); int64_t k = P_(s) + i;
+  putv(s, cb, k, (float)((double)getv(s, cb, k) + v));
+}
+void wv_solve_learned_start(const Ctx& s, wv_net* net, hipStream_t st) {
+  hipLaunchKernelGGL(k_net_scale, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_net_scale, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_net_features, dim3(nb(s.N, 256)), dim3(256), 0, st, s, wv_net_input_dev(net));
+  wv_net_forward_dev(net, st);
+  const float* qf = wv_net_qf_dev(net); const float* qc = wv_net_qc_dev(net);
+  hipLaunchKernelGGL(k_ws_dots, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, qf, qc, 0);
+  hipLaunchKernelGGL(k_ws_dots, dim3(1), dim3(WV_RED), 0, st, s, qf, qc, 1);
+  hipLaunchKernelGGL(k_ws_update, dim3(nb(s.N, 256)), dim3(256), 0, st, s, qf, qc);
+  hipLaunchKernelGGL(k_init_given, dim3(nb(s.N, 256)), dim3(256), 0, st, s);   // r = b - A p
 }

@@ -32,6 +32,11 @@ EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_reques
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
            "wv_levels", "wv_time_kernel", "wv_pressure_solve")
+#: every entry point declared in include/waves_net.h
+NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
+               "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
+               "wv_net_train", "wv_net_eval", "wv_net_last_error", "wv_solver_attach_net")
+SOLVE_ZERO_MEAN, SOLVE_GIVEN_RHS = 1, 2
 
 
 class WavesError(RuntimeError):
@@ -94,6 +99,22 @@ def library() -> ctypes.CDLL:
         L.wv_levels.argtypes = [vp, ctypes.POINTER(i32), i32]
         L.wv_time_kernel.argtypes = [vp, i32, i32, fp]
         L.wv_pressure_solve.argtypes = [vp, i32, ctypes.c_float, i32, i32, ctypes.POINTER(_SolveReport)]
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        L.wv_net_param_count.restype = i64
+        L.wv_net_param_count.argtypes = []
+        L.wv_net_create.argtypes = [i32, i32, i32, i32, ctypes.POINTER(vp)]
+        L.wv_net_destroy.argtypes = [vp]
+        L.wv_net_destroy.restype = None
+        L.wv_net_set_params.argtypes = [vp, fp, i64]
+        L.wv_net_get_params.argtypes = [vp, fp, i64]
+        L.wv_net_forward.argtypes = [vp, u16p, fp, fp]
+        L.wv_net_loss_grad.argtypes = [vp, u16p, fp, fp, fp]
+        L.wv_net_set_dataset.argtypes = [vp, u16p, fp, i32]
+        L.wv_net_train.argtypes = [vp, i32, i32, ctypes.c_float, ctypes.c_uint64, fp]
+        L.wv_net_eval.argtypes = [vp, i32, i32, fp]
+        L.wv_net_last_error.restype = ctypes.c_char_p
+        L.wv_net_last_error.argtypes = [vp]
+        L.wv_solver_attach_net.argtypes = [vp, vp]
         _lib = L
         return L
 
@@ -235,13 +256,21 @@ class Solver:
         return float(out.value)
 
     def pressure_solve(self, mode: int = 0, tolerance: float = 1e-6, max_iterations: int = 4096,
-                       zero_mean: bool = True) -> Dict:
+                       zero_mean: bool = True, given_rhs: bool = False) -> Dict:
         """Standalone pressure solve on the current state (wv_pressure_solve):
-        mode 0 plain FP32 MGPCG, mode 1 BF16 V-cycle / FP64 dots and true residual."""
+        mode 0 plain FP32 MGPCG, mode 1 BF16 V-cycle / FP64 dots and true
+        residual, mode 2 mode 1 plus the attached network's warm start.
+        given_rhs: b is work field 5 and the start is the candidate pressure."""
         r = _SolveReport()
+        flags = (SOLVE_GIVEN_RHS if given_rhs else (SOLVE_ZERO_MEAN if zero_mean else 0))
         self._check(self._lib.wv_pressure_solve(self._h, int(mode), float(tolerance), int(max_iterations),
-                                                int(zero_mean), ctypes.byref(r)))
+                                                flags, ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in _SolveReport._fields_}
+
+    def attach_net(self, net) -> None:
+        """Use `net` (waves_amd.net.Net on this grid, batch 1) for mode 2."""
+        self._check(self._lib.wv_solver_attach_net(self._h, net._h if net is not None else None))
+        self._net = net
 
     def step(self, max_batches: int = 64) -> int:
         """Advance until one more step is accepted (or the run stops); one
