@@ -106,7 +106,166 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     }
 }
 
+// Forward convolution with the input halo staged in LDS: a block computes a
+// 16 x 4 x 4 tile of output cells; its 18 x 6 x 6 input halo (every input
+// channel, the upsampled coarse part included) is read from HBM once and the
+// 27 taps read it from LDS.  Wave w computes the rows (y, z) = (w, 0..3) of
+// the tile as four 16-cell MFMA tiles sharing each weight fragment.
+#define TX 16
+#define TY 4
+#define TZ 4
+#define HX (TX + 2)
+#define HY (TY + 2)
+#define HZ (TZ + 2)
+#define HALO (HX * HY * HZ)
+template <int CIN, int NT, bool WLDS>
+__global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lds[];
+  bf16_t* halo = lds;                                   // HALO * CIN
+  bf16_t* wl = lds + HALO * CIN;                        // NT*16 rows x kp (WLDS)
+  const int tilesz = (a.g.nz + TZ - 1) / TZ;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int z0 = (blockIdx.z % tilesz) * TZ, b = blockIdx.z / tilesz;
+  constexpr int Q = CIN / 4;                            // 8-byte chunks per halo cell
+  constexpr int TOTAL = HALO * Q, PER = 8;              // PER loads in flight per thread
+  for (int base = 0; base < TOTAL; base += 256 * PER) {
+    v4s v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = base + u * 256 + threadIdx.x;
+      v[u] = v4s{0, 0, 0, 0};
+      if (e < TOTAL) {
+        const int hc = e / Q, ch = (e - hc * Q) * 4;
+        const int hx = hc % HX, hy = (hc / HX) % HY, hz = hc / (HX * HY);
+        const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+        if (inside(a.g, x, y, z))
+          v[u] = ch < a.ca ? *(const v4s*)(a.xa + encode(a.g, x, y, z, b) * a.ca + ch)
+                           : *(const v4s*)(a.xb + encode(a.gb, x >> 1, y >> 1, z >> 1, b) * a.cb + (ch - a.ca));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = base + u * 256 + threadIdx.x;
+      if (e < TOTAL) *(v4s*)(halo + e * 4) = v[u];       // hc * CIN + ch == 4 e
+    }
+  }
+  if (WLDS) {
+    const int kq4 = a.kp / 4, wtotal = NT * 16 * kq4;
+    for (int base = 0; base < wtotal; base += 256 * PER) {
+      v4s v[PER];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int e = base + u * 256 + threadIdx.x;
+        const int row = e / kq4, k = (e - row * kq4) * 4;
+        v[u] = (e < wtotal && row < a.cout) ? *(const v4s*)(a.w + (int64_t)row * a.kp + k) : v4s{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int e = base + u * 256 + threadIdx.x;
+        if (e < wtotal) *(v4s*)(wl + e * 4) = v[u];         // row * kp + k == 4 e
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  constexpr int KREAL = 27 * CIN;
+  v4f acc[4][NT];
+  for (int r = 0; r < 4; r++) for (int t = 0; t < NT; t++) acc[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < a.kp; k0 += 16) {
+    const int k = k0 + 4 * kq;
+    v4s bv[NT];
+    for (int t = 0; t < NT; t++) {
+      const int co = t * 16 + m;
+      if (WLDS) bv[t] = *(const v4s*)(wl + co * a.kp + k);
+      else bv[t] = co < a.cout ? *(const v4s*)(a.w + (int64_t)co * a.kp + k) : v4s{0, 0, 0, 0};
+    }
+    int off = -1;
+    if (k < KREAL) {
+      const int tap = k / CIN, ch = k - tap * CIN;
+      off = ((tap / 9) * (HX * HY) + ((tap / 3) % 3) * HX + (tap % 3) + m) * CIN + ch;
+    }
+    for (int r = 0; r < 4; r++) {                       // row r of this wave: y = wave, z = r
+      v4s av = {0, 0, 0, 0};
+      if (off >= 0) av = *(const v4s*)(halo + off + (r * (HX * HY) + wave * HX) * CIN);
+      else if (k == KREAL) av = v4s{(short)0x3f80, 0, 0, 0};
+      for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[t], acc[r][t], 0, 0, 0);
+    }
+  }
+  const int y = y0 + wave;
+  for (int r = 0; r < 4; r++) {
+    const int z = z0 + r;
+    if (y >= a.g.ny || z >= a.g.nz) continue;
+    for (int t = 0; t < NT; t++)
+      for (int j = 0; j < 4; j++) {
+        const int x = x0 + 4 * kq + j, co = t * 16 + m;
+        if (x < a.g.nx && co < a.cout) {
+          float v = acc[r][t][j];
+          if (a.relu) v = v > 0.f ? v : 0.f;
+          const int64_t cc = encode(a.g, x, y, z, b);
+          if (a.yb) a.yb[cc * a.ystride + co] = f2bf(v);
+          else a.yf[cc * a.ystride + co] = v;
+        }
+      }
+  }
+}
+
+static int lds_limit() {
+  static int limit = -1;
+  if (limit < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = v;
+  }
+  return limit;
+}
+
+template <int CIN, int NT>
+static bool launch_tiled_nt(const ConvArgs& a, hipStream_t st) {
+  const dim3 grid((a.g.nx + TX - 1) / TX, (a.g.ny + TY - 1) / TY, ((a.g.nz + TZ - 1) / TZ) * a.g.B);
+  const size_t halo = (size_t)HALO * CIN * 2, wts = (size_t)NT * 16 * a.kp * 2;
+  const size_t lim = (size_t)lds_limit();
+  // weights join the halo in LDS on small grids, or where the block still fits
+  // five to a CU; on large grids a larger block would cost more occupancy than
+  // the cached weight reads it saves
+  const bool small = (size_t)grid.x * grid.y * grid.z <= 1024;
+  if (halo + wts <= lim && (small || halo + wts <= 32768)) {
+    hipLaunchKernelGGL((k_conv_tiled<CIN, NT, true>), grid, dim3(256), halo + wts, st, a);
+    return true;
+  }
+  if (halo <= lim) {
+    hipLaunchKernelGGL((k_conv_tiled<CIN, NT, false>), grid, dim3(256), halo, st, a);
+    return true;
+  }
+  return false;
+}
+template <int CIN>
+static bool launch_tiled_cin(const ConvArgs& a, hipStream_t st) {
+  switch ((a.cout + 15) / 16) {
+  case 1: return launch_tiled_nt<CIN, 1>(a, st);
+  case 2: return launch_tiled_nt<CIN, 2>(a, st);
+  case 4: return launch_tiled_nt<CIN, 4>(a, st);
+  default: return false;
+  }
+}
+// forward layers (bias column present) of every input width of the network
+static bool launch_tiled(const ConvArgs& a, hipStream_t st) {
+  if (!a.bias) return false;
+  switch (a.ca + a.cb) {
+  case 8: return launch_tiled_cin<8>(a, st);
+  case 16: return launch_tiled_cin<16>(a, st);
+  case 24: return launch_tiled_cin<24>(a, st);
+  case 32: return launch_tiled_cin<32>(a, st);
+  case 48: return launch_tiled_cin<48>(a, st);
+  case 64: return launch_tiled_cin<64>(a, st);
+  case 96: return launch_tiled_cin<96>(a, st);
+  default: return false;
+  }
+}
+
 static void launch_conv(const ConvArgs& a, hipStream_t st) {
+  if (launch_tiled(a, st)) return;
   const unsigned g = nblk(cells_of(a.g), 64);
   switch ((a.cout + 15) / 16) {
   case 1: hipLaunchKernelGGL(k_conv<1>, dim3(g), dim3(256), 0, st, a); break;
