@@ -143,6 +143,14 @@ typedef struct wv_solve_report {
  * network's two output fields span a correction of the starting pressure
  * chosen by an FP64 Galerkin projection (it cannot increase the A-norm
  * error), and CG uses the flexible (Polak-Ribiere) coefficient. */
+/* Precision of the batch program's pressure solve (no reference counterpart:
+ * the reference always runs mode 0).  0: the reference's FP32 PCG and
+ * V-cycle; 1: the V-cycle's operators in BF16 (FP32 vectors and accumulation)
+ * and every PCG dot product and scalar in FP64, the reference's
+ * post-projection check and defect correction unchanged.  Takes effect at the
+ * next wv_advance. */
+int wv_set_precision(wv_solver *s, int mode);
+
 enum {
   WV_SOLVE_ZERO_MEAN = 1,  /* shift b to zero mean over the wet cells (starts from p = 0) */
   WV_SOLVE_GIVEN_RHS = 2   /* b is work field 5 as written by the caller; start from the candidate p */

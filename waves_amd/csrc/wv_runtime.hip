@@ -169,6 +169,7 @@ struct wv_solver {
   hipGraphExec_t batch_exec = nullptr;
   hipGraph_t batch_graph = nullptr;
   int smoothing = 3, iterations = 8;
+  int precision = 0;      // batch program: 0 the reference's FP32 PCG, 1 BF16 V-cycle + FP64 dots
   bool use_graph = true;  // WAVES_HIP_GRAPH=0: launch each batch's kernels directly (tracing tools)
   int64_t history_capacity = 0;
   std::string error;
@@ -339,11 +340,18 @@ static int record_batch(wv_solver* s) {
   d(17); d(18); d(16);
   d(2); d(47); d(63); d(64);
   wv_mg_build(s->ctx, s->mg, s->stream);
+  const bool mixed = s->precision == 1;
+  if (mixed) wv_mg_to_bf16(s->ctx, s->mg, s->stream);
   d(3);
-  wv_mg_cycle(s->ctx, s->mg, 0, s->smoothing, true, s->stream);  // + op 4
+  if (mixed) wv_mg_cycle_bf16(s->ctx, s->mg, 0, s->smoothing, true, true, s->stream);  // + op 4 in FP64
+  else wv_mg_cycle(s->ctx, s->mg, 0, s->smoothing, true, s->stream);                  // + op 4
   for (int it = 0; it < s->iterations; it++) {
-    d(5); d(6); d(7); d(8);
-    wv_mg_cycle(s->ctx, s->mg, 1, s->smoothing, false, s->stream);  // + op 9
+    d(5);
+    if (mixed) wv_solve_dot_alpha64(s->ctx, s->stream);   // ops 6 + 7 in FP64
+    else { d(6); d(7); }
+    d(8);
+    if (mixed) wv_mg_cycle_bf16(s->ctx, s->mg, 1, s->smoothing, false, true, s->stream);  // + op 9 in FP64
+    else wv_mg_cycle(s->ctx, s->mg, 1, s->smoothing, false, s->stream);                  // + op 9
     d(10);
   }
   d(11); d(65); d(47);
@@ -627,5 +635,16 @@ extern "C" int wv_solver_attach_net(wv_solver* s, wv_net* net) {
       return fail(s, WV_EINVAL, "network grid differs from the solver grid or batch != 1");
   }
   s->net = net;
+  return WV_OK;
+}
+
+extern "C" int wv_set_precision(wv_solver* s, int mode) {
+  if (!s || mode < 0 || mode > 1) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  if (mode == s->precision) return WV_OK;
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  if (s->batch_exec) { HIPCHECK(s, hipGraphExecDestroy(s->batch_exec)); s->batch_exec = nullptr; }
+  if (s->batch_graph) { HIPCHECK(s, hipGraphDestroy(s->batch_graph)); s->batch_graph = nullptr; }
+  s->precision = mode;
   return WV_OK;
 }
