@@ -71,7 +71,10 @@ __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gm
 __device__ __forceinline__ int iclamp(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 __device__ __forceinline__ float fract(float x) { return x - floorf(x); }
 __device__ __forceinline__ float gsign(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
-__device__ __forceinline__ int pidx(int q, int n) { return q >= 0 ? q % n : n - 1 - ((-1 - q) % n); }
+__device__ __forceinline__ int pidx(int q, int n) {
+  if ((unsigned)q < (unsigned)n) return q;   // in range: the modulo is the identity
+  return q >= 0 ? q % n : n - 1 - ((-1 - q) % n);
+}
 
 __device__ __forceinline__ float hc(const Ctx& s, int a) { return s.h[a]; }
 __device__ __forceinline__ int64_t U_(const Ctx& s) { return s.N; }
@@ -111,16 +114,19 @@ __device__ __forceinline__ I3 wrap_cell(const Ctx& s, I3 q) {
 }
 __device__ __forceinline__ bool outside(const Ctx& s, I3 q) { return q.x < 0 || q.x >= s.nx || q.y < 0 || q.y >= s.ny; }
 // pressure_core.glslinc:87-96; x < 0 is the open ocean of the coastal program
-__device__ __forceinline__ bool solid(const Ctx& s, I3 q) {
-  q = wrap_cell(s, q);
-  if (q.y < 0 || q.x >= s.nx) return true;
-  if (q.y >= s.ny) return false;
+__device__ __forceinline__ bool solid_formula(const Ctx& s, I3 q) {
   if (s.slope > 0.0f) {
     F3 cc = center(s, q);
     float bed = -0.45f + gmax(0.0f, cc.x - 1.5f) / s.slope;
     return cc.y <= bed;
   }
   return false;
+}
+__device__ __forceinline__ bool solid(const Ctx& s, I3 q) {
+  q = wrap_cell(s, q);
+  if (q.y < 0 || q.x >= s.nx) return true;
+  if (q.y >= s.ny) return false;
+  return solid_formula(s, q);
 }
 __device__ __forceinline__ bool wet(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && s.geo[index3(q, dims(s))].y > 0.5f; }
 __device__ __forceinline__ float phi(const Ctx& s, I3 q) { q = wrap_cell(s, q); return outside(s, q) ? center(s, q).y - eta(s, q) : s.geo[index3(q, dims(s))].x; }

@@ -45,8 +45,27 @@ __device__ __forceinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int com
   }
   q = selected; return best < 1e19f;
 }
-// coupled_motion.glslinc:86-111
+// solid() for a cell whose z is already in [0, nz) without periodic x, where
+// the periodic wrap is the identity
+__device__ __forceinline__ bool solid_direct(const Ctx& s, I3 q) {
+  if (q.y < 0 || q.x >= s.nx) return true;
+  if (q.y >= s.ny) return false;
+  return solid_formula(s, q);
+}
+// coupled_motion.glslinc:86-111.  The first branch is the common case of the
+// general code below, taken without its reflections and periodic wraps: a face
+// inside the grid (x >= 0, no periodic x, z and its left neighbour in range)
+// between two non-solid cells reads the stored value with parity 1.
 __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bool forward, bool& valid) {
+  if (s.perx < .5f && q.x >= 0 && q.y >= 0 && q.z >= (axis == 2 ? 1 : 0) && q.z < s.nz &&
+      q.x < s.nx + (axis == 0 ? 1 : 0) && q.y < s.ny + (axis == 1 ? 1 : 0)) {
+    I3 l = q; addc(l, axis, -1);
+    if (!solid_direct(s, l) && !solid_direct(s, q)) {
+      int acc = accepted(s);
+      int64_t f = face_local(s, q, axis); valid = s.motion[valid_base(s, acc) + f] > .5f;
+      return forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
+    }
+  }
   float parity = 1;
   if (q.y < 0) { q.y = -q.y - (axis == 1 ? 0 : 1); if (axis == 1) parity = -parity; }
   if (s.perx < .5f) {
@@ -68,7 +87,21 @@ __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bo
   int64_t f = face_local(s, q, axis); valid = s.motion[valid_base(s, acc) + f] > .5f;
   return parity * (forward ? s.motion[forward_base(s) + f] : getv(s, acc, face_index(s, q, axis)));
 }
-// coupled_motion.glslinc:112-137: staggered trilinear sample in fma difference form
+// one lattice corner of a staggered sample: value and support of the face at lo + (xx, y, z)
+__device__ __forceinline__ float sample_tap(const Ctx& s, I3 lo, int xx, int y, int z, int axis, bool forward, bool interior,
+                                            I3 fn, int acc, bool& supported) {
+  if (interior) {
+    int64_t f = face_local(s, lo, axis) + (xx + (int64_t)fn.x * (y + (int64_t)fn.y * z));
+    supported = s.motion[valid_base(s, acc) + f] > .5f;
+    return forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
+  }
+  return velocity_donor(s, mk(lo.x + xx, lo.y + y, lo.z + z), axis, forward, supported);
+}
+// coupled_motion.glslinc:112-137: staggered trilinear sample in fma difference
+// form.  The eight corners are visited as four x-pairs in a loop that is not
+// unrolled (one copy of the donor code per kernel instead of eight); the
+// corner order, the support and limit updates and the interpolation are the
+// reference's.
 __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
   F3 lattice = mkf(x.x / s.h[0] - .5f, x.y / s.h[1] - .5f, x.z / s.h[2] - .5f);
   addc(lattice, axis, .5f);
@@ -79,26 +112,47 @@ __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis,
                   !solid(s, mk(lo.x + 1, lo.y - 1, lo.z));
   I3 fn = dims(s); addc(fn, axis, 1);
   int acc = accepted(s);
-  float values[8];
-#pragma unroll
-  for (int c8 = 0; c8 < 8; c8++) {
-    int xx = c8 & 1, y = (c8 >> 1) & 1, z = c8 >> 2;
-    bool supported; float value;
-    if (interior) {
-      int64_t f = face_local(s, lo, axis) + (xx + (int64_t)fn.x * (y + (int64_t)fn.y * z));
-      supported = s.motion[valid_base(s, acc) + f] > .5f;
-      value = forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
-    } else value = velocity_donor(s, mk(lo.x + xx, lo.y + y, lo.z + z), axis, forward, supported);
-    values[c8] = value;
-    float wx = (1 - t.x) * (1 - (float)xx) + t.x * (float)xx;
+  float a = 0, b = 0, d = 0, e = 0;
+#pragma unroll 1
+  for (int pr = 0; pr < 4; pr++) {
+    int y = pr & 1, z = pr >> 1;
+    bool s0, s1;
+    float v0 = sample_tap(s, lo, 0, y, z, axis, forward, interior, fn, acc, s0);
+    float v1 = sample_tap(s, lo, 1, y, z, axis, forward, interior, fn, acc, s1);
     float wy = (1 - t.y) * (1 - (float)y) + t.y * (float)y;
     float wz = (1 - t.z) * (1 - (float)z) + t.z * (float)z;
-    if (wx * wy * wz > 0) { valid = valid && supported; lim_lo = gmin(lim_lo, value); lim_hi = gmax(lim_hi, value); }
+    float wx0 = (1 - t.x) * (1 - (float)0) + t.x * (float)0;
+    float wx1 = (1 - t.x) * (1 - (float)1) + t.x * (float)1;
+    if (wx0 * wy * wz > 0) { valid = valid && s0; lim_lo = gmin(lim_lo, v0); lim_hi = gmax(lim_hi, v0); }
+    if (wx1 * wy * wz > 0) { valid = valid && s1; lim_lo = gmin(lim_lo, v1); lim_hi = gmax(lim_hi, v1); }
+    float r = fmaf(t.x, v1 - v0, v0);
+    a = pr == 0 ? r : a; b = pr == 1 ? r : b; d = pr == 2 ? r : d; e = pr == 3 ? r : e;
   }
-  float a = fmaf(t.x, values[1] - values[0], values[0]), b = fmaf(t.x, values[3] - values[2], values[2]);
-  float d = fmaf(t.x, values[5] - values[4], values[4]), e = fmaf(t.x, values[7] - values[6], values[6]);
   float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
   return fmaf(t.z, upper - lower, lower);
+}
+// The samples of one RK2 trace of step h from x (coupled_motion.glslinc:168-188):
+// velocity at x (or the cached sv), at mid = x - 0.5 h u(x), and every component
+// at end = x - h u(mid), `component`'s from the forward field if forward_end.
+// One loop over the nine samples keeps a single copy of the sampling code.
+struct Rk2Trace { F3 sv, um, mid, end; float value, l0, l1; bool vs, vm, ve; };
+__device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int component, bool forward_end, bool have_sv, F3 sv) {
+  Rk2Trace t;
+  t.sv = sv; t.um = mkf(0, 0, 0); t.end = x; t.value = 0; t.l0 = 0; t.l1 = 0; t.vs = true; t.vm = true; t.ve = true;
+  t.mid = have_sv ? mkf(x.x - .5f * h * sv.x, x.y - .5f * h * sv.y, x.z - .5f * h * sv.z) : x;
+  F3 p = t.mid;
+#pragma unroll 1
+  for (int it = have_sv ? 3 : 0; it < 9; it++) {
+    int stage = it / 3, ax = it - 3 * stage;
+    bool ok; float lo, hi;
+    float v = sampled_component(s, p, ax, stage == 2 && ax == component && forward_end, ok, lo, hi);
+    if (stage == 0) { setc(t.sv, ax, v); t.vs = t.vs && ok; }
+    else if (stage == 1) { setc(t.um, ax, v); t.vm = t.vm && ok; }
+    else { t.ve = t.ve && ok; if (ax == component) { t.value = v; t.l0 = lo; t.l1 = hi; } }
+    if (ax == 2 && stage == 0) { t.mid = mkf(x.x - .5f * h * t.sv.x, x.y - .5f * h * t.sv.y, x.z - .5f * h * t.sv.z); p = t.mid; }
+    if (ax == 2 && stage == 1) { t.end = mkf(x.x - h * t.um.x, x.y - h * t.um.y, x.z - h * t.um.z); p = t.end; }
+  }
+  return t;
 }
 __device__ __forceinline__ F3 sampled_velocity_checked(const Ctx& s, F3 x, bool& supported) {
   F3 v; supported = true;

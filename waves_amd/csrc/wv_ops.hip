@@ -284,7 +284,9 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
 }
 // ---------------------------------------------- MacCormack (27 / 28) --------
 // coupled_motion.glslinc:168-188, 409-426 (transport-cache and plain forms)
-template <bool CORRECT>
+// CACHE mirrors numerics flag 4 (set for the whole run): one kernel per form
+// keeps each variant's registers and code to its own path
+template <bool CORRECT, bool CACHE>
 __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID(); if (i >= s.NF) return;
@@ -294,30 +296,22 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
   int acc = accepted(s); float dt = timestep(s);
   float value = 0;
   if (!wall_face(s, q, axis) && ((s.flags & 2) == 0 || s.motion[valid_base(s, acc) + i] > .5f)) {
-    if (s.flags & 4) {
+    if (CACHE) {
       int64_t base = trace_base(s), nf = s.NF;
       if (!CORRECT) {
-        bool vs, vm, ve; float l0, l1;
-        F3 sv = sampled_velocity_checked(s, x, vs);
-        F3 mid = mkf(x.x - .5f * dt * sv.x, x.y - .5f * dt * sv.y, x.z - .5f * dt * sv.z);
-        F3 um = sampled_velocity_checked(s, mid, vm);
-        F3 end = mkf(x.x - dt * um.x, x.y - dt * um.y, x.z - dt * um.z);
-        value = checked_endpoint(s, end, axis, false, ve, l0, l1);
-        s.motion[base + i] = l0; s.motion[base + nf + i] = l1;
-        s.motion[base + 2 * nf + i] = vs && vm && ve && trace_points_clear(s, x, mid, end) ? 1 : 0;
-        s.motion[base + 3 * nf + i] = sv.x; s.motion[base + 4 * nf + i] = sv.y; s.motion[base + 5 * nf + i] = sv.z;
+        Rk2Trace t = rk2_samples(s, x, dt, axis, false, false, mkf(0, 0, 0));
+        value = t.value;
+        s.motion[base + i] = t.l0; s.motion[base + nf + i] = t.l1;
+        s.motion[base + 2 * nf + i] = t.vs && t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) ? 1 : 0;
+        s.motion[base + 3 * nf + i] = t.sv.x; s.motion[base + 4 * nf + i] = t.sv.y; s.motion[base + 5 * nf + i] = t.sv.z;
       } else {
         float forward = s.motion[forward_base(s) + i];
         if (s.motion[base + 2 * nf + i] < .5f) value = forward;
         else {
           F3 sv = mkf(s.motion[base + 3 * nf + i], s.motion[base + 4 * nf + i], s.motion[base + 5 * nf + i]);
-          bool vm, ve; float u0, u1; float rdt = -dt;
-          F3 mid = mkf(x.x - .5f * rdt * sv.x, x.y - .5f * rdt * sv.y, x.z - .5f * rdt * sv.z);
-          F3 um = sampled_velocity_checked(s, mid, vm);
-          F3 end = mkf(x.x - rdt * um.x, x.y - rdt * um.y, x.z - rdt * um.z);
-          float backward = checked_endpoint(s, end, axis, true, ve, u0, u1);
-          float corrected = forward + .5f * (getv(s, acc, U_(s) + i) - backward);
-          value = vm && ve && trace_points_clear(s, x, mid, end) && corrected >= s.motion[base + i] && corrected <= s.motion[base + nf + i] ? corrected : forward;
+          Rk2Trace t = rk2_samples(s, x, -dt, axis, true, true, sv);
+          float corrected = forward + .5f * (getv(s, acc, U_(s) + i) - t.value);
+          value = t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) && corrected >= s.motion[base + i] && corrected <= s.motion[base + nf + i] ? corrected : forward;
         }
       }
     } else if (!CORRECT) {
@@ -757,8 +751,8 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 23: L256(k_reconstruct<1>, s.N); break;
   case 24: L256(k_flux, s.NF); break;
   case 25: LRED(k_vof_update, 4); break;
-  case 27: L256(k_maccormack<false>, s.NF); break;
-  case 28: L256(k_maccormack<true>, s.NF); break;
+  case 27: if (s.flags & 4) L256((k_maccormack<false, true>), s.NF); else L256((k_maccormack<false, false>), s.NF); break;
+  case 28: if (s.flags & 4) L256((k_maccormack<true, true>), s.NF); else L256((k_maccormack<true, false>), s.NF); break;
   case 30: L256(k_ext_seed, s.NF); break;
   case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 256)), dim3(256), 0, st, s, (op - 31) % 2); break;
   case 34: L256(k_ext_fill, s.NF); break;
