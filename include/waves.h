@@ -111,8 +111,12 @@ int wv_levels(wv_solver *s, int *meta16, int max_levels);
 /* Average device duration [us] of one launch of a hot kernel, timed with HIP
  * events on the solver's stream over `reps` launches on the current state
  * (kernel: 0 = level-0 Chebyshev sweep, 1 = fine masked 7-point SpMV,
- * 2 = level-0 residual, 3 = plain copy of N floats, the bandwidth reference).
- * Writes only V-cycle / PCG / Galerkin scratch: the solver state is unchanged. */
+ * 2 = level-0 residual, 3 = plain copy of N floats, the bandwidth reference,
+ * 4 = Chebyshev sweep on MG level 1, 5 = MacCormack forward trace, op 27;
+ * kernels 4 and 5 run against a copy of the control vector with a trial open).
+ * Writes only scratch that every trial recomputes before use (V-cycle / PCG /
+ * Galerkin vectors, the MacCormack forward field and trace cache): the
+ * accepted state is unchanged. */
 int wv_time_kernel(wv_solver *s, int kernel, int reps, float *avg_us);
 
 /* Report of one standalone pressure solve. */

@@ -60,9 +60,13 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
     const float* row = m.rows + L.row_off + rc.row;
     const uint16_t* row16 = m.rows16 + L.row_off + rc.row;
     const float* v = m.vec + L.vec_off + (int64_t)field * N;
+    // unrolled so several coefficient and neighbour loads are in flight; the
+    // additions stay in k order
     if (interior) {
+#pragma unroll 8
       for (int k = rc.lane; k < W; k += rc.G) sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[rc.row + sh_delta[k]];
     } else {
+#pragma unroll 4
       for (int k = rc.lane; k < W; k += rc.G) {
         int j = mwrap(m, l, mk(q.x + sh_off[3 * k], q.y + sh_off[3 * k + 1], q.z + sh_off[3 * k + 2]));
         sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[j >= 0 ? j : 0];
@@ -493,12 +497,21 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
       I3 q = coord(i, nf); int qa[3] = {q.x, q.y, q.z};
       int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
       const float* T = m.ap + i;
-      for (int b = 0; b < E; b++) {
+      // box entry b = bx + E0 (by + E1 bz); its coarse node's offset from I is
+      // taken from the parent coordinates directly (mdelta of the encoded node)
+      I3 nc = mdims(m, lc);
+      int b = 0;
+      for (int bz = 0; bz < E2; bz++) for (int by = 0; by < E1; by++) for (int bx = 0; bx < E0; bx++, b++) {
         float v = T[(int64_t)b * F.count];
         if (v == 0) continue;
-        int P[3] = {anchor[0] + b % E0, anchor[1] + (b / E0) % E1, anchor[2] + b / (E0 * E1)};
-        int J = parent_node(m, lc, P);
-        I3 d = mdelta(m, lc, (int)I, J);
+        int y = anchor[1] + by;
+        if (y >= nc.y) { s.c[7] = 31; continue; }   // parent_node's dropped node: no slot
+        if (y < 0) y = 0;
+        int x = m.perx ? pmod(anchor[0] + bx, nc.x) : iclamp(anchor[0] + bx, 0, nc.x - 1);
+        int z = pmod(anchor[2] + bz, nc.z);
+        I3 d = mk(x - qc.x, y - qc.y, z - qc.z);
+        if (m.perx) d.x = pmod(d.x + nc.x / 2, nc.x) - nc.x / 2;
+        d.z = pmod(d.z + nc.z / 2, nc.z) - nc.z / 2;
         int sl = in_box2(d) ? slots[slot125(d)] : -1;
         if (sl < 0) { s.c[7] = 31; continue; }
         acc[sl * GAL_THREADS + tid] += wI * v;
@@ -655,6 +668,11 @@ __global__ void __launch_bounds__(256) k_copy_timed(const float* __restrict__ sr
   int64_t i = GID(); if (i < n) dst[i] = src[i];
 }
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
+  if (kernel == 4) {   // Chebyshev sweep 2 on level 1 (the largest coarse level), gates open
+    const int l = m.L > 1 ? 1 : 0;
+    hipLaunchKernelGGL((k_mg_coarse<0, false>), dim3(nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256)), dim3(256), 0, st, c, m, l, 1, 2, 1);
+    return;
+  }
   if (kernel == 3) { hipLaunchKernelGGL(k_copy_timed, dim3(nblocks(c.N, 256)), dim3(256), 0, st, c.work + 3 * c.N, m.ap, c.N); return; }
   if (kernel == 0) hipLaunchKernelGGL(k_mg_cheb_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);
   else if (kernel == 2) hipLaunchKernelGGL(k_mg_residual_timed, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, c, m);

@@ -5,7 +5,10 @@
 
 #define GID() ((int64_t)blockIdx.x * blockDim.x + threadIdx.x)
 
-__device__ __forceinline__ int pmod(int x, int n) { return x >= 0 ? x % n : (n - 1) - ((-1 - x) % n); }
+__device__ __forceinline__ int pmod(int x, int n) {
+  if ((unsigned)x < (unsigned)n) return x;   // in range: the modulo is the identity
+  return x >= 0 ? x % n : (n - 1) - ((-1 - x) % n);
+}
 __device__ __forceinline__ int64_t at(const MGDev& m, int l, int field, int64_t i) { return m.lv[l].vec_off + (int64_t)field * m.lv[l].count + i; }
 __device__ __forceinline__ I3 mdims(const MGDev& m, int l) { return mk(m.lv[l].n[0], m.lv[l].n[1], m.lv[l].n[2]); }
 // pressure_multigrid.glslinc:36-47

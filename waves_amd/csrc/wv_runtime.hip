@@ -160,6 +160,7 @@ struct wv_solver {
   float* quota_dev = nullptr;
   float* quota_host = nullptr;   // pinned
   float* ctrl_host = nullptr;    // pinned, readback slots
+  float* ctl_timing = nullptr;   // device control vector used only by wv_time_kernel
   wv_net* net = nullptr;         // learned warm start (not owned)
   hipEvent_t ctrl_event = nullptr;
   std::atomic<double> quota{0.0};
@@ -256,6 +257,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.dctl, 16);
   ALLOC(c.pacc, c.N);
   ALLOC(c.aux, 2 * c.N);
+  ALLOC(s->ctl_timing, WV_NCTRL);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
   MGDev& m = s->mg;
@@ -524,13 +526,28 @@ extern "C" int wv_levels(wv_solver* s, int* meta16, int max_levels) {
 
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st);
 extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us) {
-  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 3) return WV_EINVAL;
+  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 5) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
+  Ctx c = s->ctx;
+  if (kernel >= 4) {
+    // the step kernels gate themselves on the control vector: time them against
+    // a copy of it with a trial open (c[4] = c[5] = 1, no convergence, no failure)
+    float host[WV_NCTRL];
+    HIPCHECK(s, hipStreamSynchronize(s->stream));
+    HIPCHECK(s, hipMemcpy(host, s->ctx.c, sizeof host, hipMemcpyDeviceToHost));
+    host[4] = 1; host[5] = 1; host[6] = 0; host[7] = 0; host[57] = 0;
+    HIPCHECK(s, hipMemcpy(s->ctl_timing, host, sizeof host, hipMemcpyHostToDevice));
+    c.c = s->ctl_timing;
+  }
+  auto launch = [&]() {
+    if (kernel == 5) wv_launch_op(c, 27, s->stream);   // MacCormack forward trace
+    else wv_launch_timed(c, s->mg, kernel, s->stream);
+  };
   hipEvent_t a, b;
   HIPCHECK(s, hipEventCreate(&a)); HIPCHECK(s, hipEventCreate(&b));
-  wv_launch_timed(s->ctx, s->mg, kernel, s->stream);  // warm
+  launch();  // warm
   HIPCHECK(s, hipEventRecord(a, s->stream));
-  for (int r = 0; r < reps; r++) wv_launch_timed(s->ctx, s->mg, kernel, s->stream);
+  for (int r = 0; r < reps; r++) launch();
   HIPCHECK(s, hipEventRecord(b, s->stream));
   HIPCHECK(s, hipEventSynchronize(b));
   float ms = 0; HIPCHECK(s, hipEventElapsedTime(&ms, a, b));
