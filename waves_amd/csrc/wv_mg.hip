@@ -460,8 +460,13 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l) {
       acc[b * 128 + tid] += aij * w;
     }
   }
-  float* T = m.ap + i;
-  for (int b = 0; b < E; b++) T[(int64_t)b * F.count] = acc[b * 128 + tid];
+  // (A P)_i is stored row-major, padded to a multiple of 4 floats, so that the
+  // second stage reads a fine row's box with 16-byte loads
+  const int Ep = (E + 3) & ~3;
+  float4* T = (float4*)(m.ap + i * Ep);
+  for (int b = 0; b < Ep; b += 4)
+    T[b >> 2] = make_float4(acc[b * 128 + tid], b + 1 < E ? acc[(b + 1) * 128 + tid] : 0.f,
+                            b + 2 < E ? acc[(b + 2) * 128 + tid] : 0.f, b + 3 < E ? acc[(b + 3) * 128 + tid] : 0.f);
 }
 #define GAL_THREADS 64
 __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m, int l) {
@@ -496,13 +501,15 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
       if (wI == 0) continue;
       I3 q = coord(i, nf); int qa[3] = {q.x, q.y, q.z};
       int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
-      const float* T = m.ap + i;
+      const float4* T = (const float4*)(m.ap + (int64_t)i * ((E + 3) & ~3));
       // box entry b = bx + E0 (by + E1 bz); its coarse node's offset from I is
       // taken from the parent coordinates directly (mdelta of the encoded node)
       I3 nc = mdims(m, lc);
       int b = 0;
+      float4 quad = make_float4(0.f, 0.f, 0.f, 0.f);
       for (int bz = 0; bz < E2; bz++) for (int by = 0; by < E1; by++) for (int bx = 0; bx < E0; bx++, b++) {
-        float v = T[(int64_t)b * F.count];
+        if ((b & 3) == 0) quad = T[b >> 2];
+        float v = (b & 3) == 0 ? quad.x : (b & 3) == 1 ? quad.y : (b & 3) == 2 ? quad.z : quad.w;
         if (v == 0) continue;
         int y = anchor[1] + by;
         if (y >= nc.y) { s.c[7] = 31; continue; }   // parent_node's dropped node: no slot

@@ -134,7 +134,7 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
       F.box[a] = mx; e *= mx;
     }
     if (e > 64) return false;  // stage-1 LDS holds 64 box entries per fine row
-    p.ap_floats = std::max(p.ap_floats, e * F.count);
+    p.ap_floats = std::max(p.ap_floats, ((e + 3) / 4 * 4) * F.count);  // rows padded to 4 floats
   }
   // lanes per row: enough threads to fill the chip on small levels
   for (int l = 0; l < p.L; l++) {
