@@ -147,6 +147,11 @@ typedef struct wv_solve_report {
  * network's two output fields span a correction of the starting pressure
  * chosen by an FP64 Galerkin projection (it cannot increase the A-norm
  * error), and CG uses the flexible (Polak-Ribiere) coefficient. */
+/* Liquid volume sum_i F_i hx hy hz of bank 0 or 1, reduced in FP64 on the
+ * device in a fixed order (no reference counterpart: the reference's volume,
+ * c[17], is an FP32 sum).  Waits for the handle's stream. */
+int wv_liquid_volume(wv_solver *s, int bank, double *volume);
+
 /* Precision of the batch program's pressure solve (no reference counterpart:
  * the reference always runs mode 0).  0: the reference's FP32 PCG and
  * V-cycle; 1: the V-cycle's operators in BF16 (FP32 vectors and accumulation)

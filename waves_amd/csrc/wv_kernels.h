@@ -55,5 +55,6 @@ void wv_solve_dot_alpha64(const Ctx& s, hipStream_t st);
 void wv_solve_true_residual64(const Ctx& s, const MGDev& m, bool replace, int parts, hipStream_t st);
 void wv_solve_finish(const Ctx& s, hipStream_t st);
 void wv_solve_init_given(const Ctx& s, hipStream_t st);
+void wv_solve_volume64(const Ctx& s, int bank, hipStream_t st);
 struct wv_net;
 void wv_solve_learned_start(const Ctx& s, wv_net* net, hipStream_t st);

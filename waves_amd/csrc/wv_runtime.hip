@@ -665,3 +665,14 @@ extern "C" int wv_set_precision(wv_solver* s, int mode) {
   s->precision = mode;
   return WV_OK;
 }
+
+extern "C" int wv_liquid_volume(wv_solver* s, int bank, double* volume) {
+  if (!s || !volume || (bank != 0 && bank != 1)) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  wv_solve_volume64(s->ctx, bank, s->stream);
+  double d[16];
+  HIPCHECK(s, hipMemcpyAsync(d, s->ctx.dctl, sizeof d, hipMemcpyDeviceToHost, s->stream));
+  HIPCHECK(s, hipStreamSynchronize(s->stream));
+  *volume = d[13];
+  return WV_OK;
+}

@@ -326,3 +326,21 @@ void wv_solve_learned_start(const Ctx& s, wv_net* net, hipStream_t st) {
   hipLaunchKernelGGL(k_ws_update, dim3(nb(s.N, 256)), dim3(256), 0, st, s, qf, qc);
   hipLaunchKernelGGL(k_init_given, dim3(nb(s.N, 256)), dim3(256), 0, st, s);   // r = b - A p
 }
+
+// Global liquid volume sum_i F_i h_x h_y h_z of one bank in FP64 (the FP32
+// volume of the acceptance op, pressure_pass.glslinc:78-103, is c[17]);
+// deterministic: fixed 128-lane groups, then one group -> dctl[13].
+__global__ void __launch_bounds__(WV_RED) k_volume64(Ctx s, int bank, int final) {
+  int64_t N = s.N;
+  if (!final) {
+    int64_t i = GID();
+    d4 v = mkd(i < N ? (double)getv(s, bank, i) : 0.0, 0, 0, 0);
+    v = group_reduce_d(v); put_partial(s, v); return;
+  }
+  d4 g = global_reduce_d(s.partial64, (N + WV_RED - 1) / WV_RED);
+  if (threadIdx.x == 0) s.dctl[13] = g.x * ((double)s.h[0] * (double)s.h[1] * (double)s.h[2]);
+}
+void wv_solve_volume64(const Ctx& s, int bank, hipStream_t st) {
+  hipLaunchKernelGGL(k_volume64, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, bank, 0);
+  hipLaunchKernelGGL(k_volume64, dim3(1), dim3(WV_RED), 0, st, s, bank, 1);
+}

@@ -31,7 +31,7 @@ FIELD_PRESSURE64 = 18
 EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_request_time",
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
-           "wv_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision")
+           "wv_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision", "wv_liquid_volume")
 #: every entry point declared in include/waves_net.h
 NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
                "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
@@ -100,6 +100,7 @@ def library() -> ctypes.CDLL:
         L.wv_time_kernel.argtypes = [vp, i32, i32, fp]
         L.wv_pressure_solve.argtypes = [vp, i32, ctypes.c_float, i32, i32, ctypes.POINTER(_SolveReport)]
         L.wv_set_precision.argtypes = [vp, i32]
+        L.wv_liquid_volume.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
         u16p = ctypes.POINTER(ctypes.c_uint16)
         L.wv_net_param_count.restype = i64
         L.wv_net_param_count.argtypes = []
@@ -267,6 +268,14 @@ class Solver:
         self._check(self._lib.wv_pressure_solve(self._h, int(mode), float(tolerance), int(max_iterations),
                                                 flags, ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in _SolveReport._fields_}
+
+    def liquid_volume(self, bank: Optional[int] = None) -> float:
+        """FP64 liquid volume of a bank (default: the accepted one)."""
+        if bank is None:
+            bank = int(self.control[0])
+        v = ctypes.c_double()
+        self._check(self._lib.wv_liquid_volume(self._h, int(bank), ctypes.byref(v)))
+        return v.value
 
     def set_precision(self, mode: int) -> None:
         """Batch program precision: 0 the reference's FP32 PCG, 1 BF16 V-cycle
