@@ -170,6 +170,8 @@ def wet_operator(o):
                 q = [x, y, z]
                 q[axis] += sign
                 q[2] %= nz
+                if o.settings.periodic_x:
+                    q[0] %= nx
                 k = float(work[6 + face, i])
                 A[idx[i], idx[i]] += k
                 if 0 <= q[0] < nx and 0 <= q[1] < ny:
@@ -216,7 +218,8 @@ def mg_pair_after_build(settings):
 def dense_transfer(o, l, active_c):
     """P (fine level l -> coarse l+1) from the oracle's own transfer rules,
     recomputed densely in FP64 from the level descriptors (trilinear with
-    clamped / periodic ends, aggregation below aggregate_from)."""
+    clamped ends, periodic in z and, with periodic_x, in x; aggregation below
+    aggregate_from)."""
     levels = o.levels()
     nf = levels[l][:3]
     f = levels[l][5:8]
@@ -248,7 +251,7 @@ def dense_transfer(o, l, active_c):
                         continue
                     nb = max(nb, 0)
                 elif a == 0:
-                    nb = min(max(nb, 0), nc[0] - 1)
+                    nb = nb % nc[0] if o.settings.periodic_x else min(max(nb, 0), nc[0] - 1)
                 else:
                     nb %= nc[2]
                 opts.append((nb, w))
