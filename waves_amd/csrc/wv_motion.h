@@ -88,9 +88,9 @@ __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bo
   return parity * (forward ? s.motion[forward_base(s) + f] : getv(s, acc, face_index(s, q, axis)));
 }
 // one lattice corner of a staggered sample: value and support of the face at lo + (xx, y, z)
-__device__ __forceinline__ float sample_tap(const Ctx& s, I3 lo, int xx, int y, int z, int axis, bool forward, bool interior,
+__device__ __forceinline__ float sample_tap(const Ctx& s, I3 lo, int xx, int y, int z, int axis, bool forward, bool direct,
                                             I3 fn, int acc, bool& supported) {
-  if (interior) {
+  if (direct) {
     int64_t f = face_local(s, lo, axis) + (xx + (int64_t)fn.x * (y + (int64_t)fn.y * z));
     supported = s.motion[valid_base(s, acc) + f] > .5f;
     return forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
@@ -98,18 +98,64 @@ __device__ __forceinline__ float sample_tap(const Ctx& s, I3 lo, int xx, int y, 
   return velocity_donor(s, mk(lo.x + xx, lo.y + y, lo.z + z), axis, forward, supported);
 }
 // coupled_motion.glslinc:112-137: staggered trilinear sample in fma difference
-// form.  The eight corners are visited as four x-pairs in a loop that is not
-// unrolled (one copy of the donor code per kernel instead of eight); the
+// form.  sample_at places the sample on the component's lattice and decides
+// whether all eight corners are direct reads; sample_direct then issues the
+// sixteen loads together, the general path visits the corners as four
+// x-pairs in a loop that is not unrolled (one copy of the donor code).  The
 // corner order, the support and limit updates and the interpolation are the
-// reference's.
-__device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+// reference's on both paths.
+struct SampleAt { I3 lo; F3 t; bool direct; };
+__device__ __forceinline__ SampleAt sample_at(const Ctx& s, F3 x, int axis) {
+  SampleAt a;
   F3 lattice = mkf(x.x / s.h[0] - .5f, x.y / s.h[1] - .5f, x.z / s.h[2] - .5f);
   addc(lattice, axis, .5f);
   I3 lo = mk((int)floorf(lattice.x), (int)floorf(lattice.y), (int)floorf(lattice.z));
-  F3 t = mkf(fract(lattice.x), fract(lattice.y), fract(lattice.z));
-  lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
+  a.lo = lo;
+  a.t = mkf(fract(lattice.x), fract(lattice.y), fract(lattice.z));
   bool interior = (s.flags & 8) != 0 && lo.x >= 1 && lo.y >= 1 && lo.z >= 1 && lo.x + 1 < s.nx && lo.y + 1 < s.ny && lo.z + 1 < s.nz &&
                   !solid(s, mk(lo.x + 1, lo.y - 1, lo.z));
+  // All eight corners on velocity_donor's first branch (in range, no periodic
+  // x, both cells of every corner face not solid): the bed test is monotone
+  // in y, so the lowest cell of each touched column decides for the column.
+  bool fluid = s.perx < .5f && lo.x >= 0 && lo.y >= 0 && lo.z >= (axis == 2 ? 1 : 0) && lo.z + 1 < s.nz &&
+               lo.x + 1 < s.nx + (axis == 0 ? 1 : 0) && lo.y + 1 < s.ny + (axis == 1 ? 1 : 0);
+  if (fluid) {
+    int ylow = lo.y - (axis == 1 ? 1 : 0);
+    for (int cx = lo.x - (axis == 0 ? 1 : 0); cx <= lo.x + 1; cx++) fluid = fluid && !solid_direct(s, mk(cx, ylow, lo.z));
+  }
+  a.direct = interior || fluid;
+  return a;
+}
+__device__ __forceinline__ void corner_update(F3 t, int c8, float v, bool sup, bool& valid, float& lim_lo, float& lim_hi) {
+  int xx = c8 & 1, y = (c8 >> 1) & 1, z = c8 >> 2;
+  float wx = (1 - t.x) * (1 - (float)xx) + t.x * (float)xx;
+  float wy = (1 - t.y) * (1 - (float)y) + t.y * (float)y;
+  float wz = (1 - t.z) * (1 - (float)z) + t.z * (float)z;
+  if (wx * wy * wz > 0) { valid = valid && sup; lim_lo = gmin(lim_lo, v); lim_hi = gmax(lim_hi, v); }
+}
+__device__ __forceinline__ float sample_direct(const Ctx& s, const SampleAt& at, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+  I3 fn = dims(s); addc(fn, axis, 1);
+  int acc = accepted(s);
+  int64_t f0 = face_local(s, at.lo, axis), vb = valid_base(s, acc), fb = forward_base(s), ub = U_(s);
+  float v[8]; bool sup[8];
+#pragma unroll
+  for (int c8 = 0; c8 < 8; c8++) {
+    int64_t f = f0 + ((c8 & 1) + (int64_t)fn.x * (((c8 >> 1) & 1) + (int64_t)fn.y * (c8 >> 2)));
+    sup[c8] = s.motion[vb + f] > .5f;
+    v[c8] = forward ? s.motion[fb + f] : getv(s, acc, ub + f);
+  }
+  lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
+#pragma unroll
+  for (int c8 = 0; c8 < 8; c8++) corner_update(at.t, c8, v[c8], sup[c8], valid, lim_lo, lim_hi);
+  F3 t = at.t;
+  float a = fmaf(t.x, v[1] - v[0], v[0]), b = fmaf(t.x, v[3] - v[2], v[2]);
+  float d = fmaf(t.x, v[5] - v[4], v[4]), e = fmaf(t.x, v[7] - v[6], v[6]);
+  float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
+  return fmaf(t.z, upper - lower, lower);
+}
+__device__ __forceinline__ float sample_general(const Ctx& s, const SampleAt& at, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+  I3 lo = at.lo; F3 t = at.t;
+  lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
   I3 fn = dims(s); addc(fn, axis, 1);
   int acc = accepted(s);
   float a = 0, b = 0, d = 0, e = 0;
@@ -117,40 +163,55 @@ __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis,
   for (int pr = 0; pr < 4; pr++) {
     int y = pr & 1, z = pr >> 1;
     bool s0, s1;
-    float v0 = sample_tap(s, lo, 0, y, z, axis, forward, interior, fn, acc, s0);
-    float v1 = sample_tap(s, lo, 1, y, z, axis, forward, interior, fn, acc, s1);
-    float wy = (1 - t.y) * (1 - (float)y) + t.y * (float)y;
-    float wz = (1 - t.z) * (1 - (float)z) + t.z * (float)z;
-    float wx0 = (1 - t.x) * (1 - (float)0) + t.x * (float)0;
-    float wx1 = (1 - t.x) * (1 - (float)1) + t.x * (float)1;
-    if (wx0 * wy * wz > 0) { valid = valid && s0; lim_lo = gmin(lim_lo, v0); lim_hi = gmax(lim_hi, v0); }
-    if (wx1 * wy * wz > 0) { valid = valid && s1; lim_lo = gmin(lim_lo, v1); lim_hi = gmax(lim_hi, v1); }
+    float v0 = sample_tap(s, lo, 0, y, z, axis, forward, at.direct, fn, acc, s0);
+    float v1 = sample_tap(s, lo, 1, y, z, axis, forward, at.direct, fn, acc, s1);
+    corner_update(t, 2 * pr, v0, s0, valid, lim_lo, lim_hi);
+    corner_update(t, 2 * pr + 1, v1, s1, valid, lim_lo, lim_hi);
     float r = fmaf(t.x, v1 - v0, v0);
     a = pr == 0 ? r : a; b = pr == 1 ? r : b; d = pr == 2 ? r : d; e = pr == 3 ? r : e;
   }
   float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
   return fmaf(t.z, upper - lower, lower);
 }
+__device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
+  SampleAt at = sample_at(s, x, axis);
+  return sample_general(s, at, axis, forward, valid, lim_lo, lim_hi);
+}
 // The samples of one RK2 trace of step h from x (coupled_motion.glslinc:168-188):
 // velocity at x (or the cached sv), at mid = x - 0.5 h u(x), and every component
 // at end = x - h u(mid), `component`'s from the forward field if forward_end.
-// One loop over the nine samples keeps a single copy of the sampling code.
+// A stage whose three samples are all direct issues their 48 loads together;
+// otherwise its samples run one at a time through the general path.
 struct Rk2Trace { F3 sv, um, mid, end; float value, l0, l1; bool vs, vm, ve; };
 __device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int component, bool forward_end, bool have_sv, F3 sv) {
   Rk2Trace t;
   t.sv = sv; t.um = mkf(0, 0, 0); t.end = x; t.value = 0; t.l0 = 0; t.l1 = 0; t.vs = true; t.vm = true; t.ve = true;
   t.mid = have_sv ? mkf(x.x - .5f * h * sv.x, x.y - .5f * h * sv.y, x.z - .5f * h * sv.z) : x;
-  F3 p = t.mid;
 #pragma unroll 1
-  for (int it = have_sv ? 3 : 0; it < 9; it++) {
-    int stage = it / 3, ax = it - 3 * stage;
-    bool ok; float lo, hi;
-    float v = sampled_component(s, p, ax, stage == 2 && ax == component && forward_end, ok, lo, hi);
-    if (stage == 0) { setc(t.sv, ax, v); t.vs = t.vs && ok; }
-    else if (stage == 1) { setc(t.um, ax, v); t.vm = t.vm && ok; }
-    else { t.ve = t.ve && ok; if (ax == component) { t.value = v; t.l0 = lo; t.l1 = hi; } }
-    if (ax == 2 && stage == 0) { t.mid = mkf(x.x - .5f * h * t.sv.x, x.y - .5f * h * t.sv.y, x.z - .5f * h * t.sv.z); p = t.mid; }
-    if (ax == 2 && stage == 1) { t.end = mkf(x.x - h * t.um.x, x.y - h * t.um.y, x.z - h * t.um.z); p = t.end; }
+  for (int stage = have_sv ? 1 : 0; stage < 3; stage++) {
+    F3 p = stage == 0 ? x : (stage == 1 ? t.mid : t.end);
+    bool f0 = stage == 2 && component == 0 && forward_end, f1 = stage == 2 && component == 1 && forward_end,
+         f2 = stage == 2 && component == 2 && forward_end;
+    SampleAt a0 = sample_at(s, p, 0), a1 = sample_at(s, p, 1), a2 = sample_at(s, p, 2);
+    float v0, v1, v2, lo0, lo1, lo2, hi0, hi1, hi2; bool ok0, ok1, ok2;
+    if (a0.direct && a1.direct && a2.direct) {
+      v0 = sample_direct(s, a0, 0, f0, ok0, lo0, hi0);
+      v1 = sample_direct(s, a1, 1, f1, ok1, lo1, hi1);
+      v2 = sample_direct(s, a2, 2, f2, ok2, lo2, hi2);
+    } else {
+      v0 = sample_general(s, a0, 0, f0, ok0, lo0, hi0);
+      v1 = sample_general(s, a1, 1, f1, ok1, lo1, hi1);
+      v2 = sample_general(s, a2, 2, f2, ok2, lo2, hi2);
+    }
+    bool ok = ok0 && ok1 && ok2;
+    if (stage == 0) { t.sv = mkf(v0, v1, v2); t.vs = ok; t.mid = mkf(x.x - .5f * h * t.sv.x, x.y - .5f * h * t.sv.y, x.z - .5f * h * t.sv.z); }
+    else if (stage == 1) { t.um = mkf(v0, v1, v2); t.vm = ok; t.end = mkf(x.x - h * t.um.x, x.y - h * t.um.y, x.z - h * t.um.z); }
+    else {
+      t.ve = ok;
+      t.value = component == 0 ? v0 : (component == 1 ? v1 : v2);
+      t.l0 = component == 0 ? lo0 : (component == 1 ? lo1 : lo2);
+      t.l1 = component == 0 ? hi0 : (component == 1 ? hi1 : hi2);
+    }
   }
   return t;
 }

@@ -136,10 +136,13 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     if (e > 64) return false;  // stage-1 LDS holds 64 box entries per fine row
     p.ap_floats = std::max(p.ap_floats, ((e + 3) / 4 * 4) * F.count);  // rows padded to 4 floats
   }
-  // lanes per row: enough threads to fill the chip on small levels
+  // lanes per row: enough threads that a sweep's load chains overlap (about
+  // 2^20 threads, 16 waves per SIMD), also on the large coarse levels
+  const char* lt = getenv("WAVES_MG_LANE_TARGET");
+  const int64_t lane_target = lt ? std::atoll(lt) : (1 << 20);
   for (int l = 0; l < p.L; l++) {
     MGLevel& L = p.lv[l]; int g = 1;
-    while (g < 32 && (int64_t)L.count * g < (1 << 18) && g * 2 <= L.width) g *= 2;
+    while (g < 32 && (int64_t)L.count * g < lane_target && g * 2 <= L.width) g *= 2;
     L.lanes = l == 0 ? 1 : g;
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
