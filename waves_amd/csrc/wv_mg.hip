@@ -85,12 +85,7 @@ __device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off
 }
 // OP 0: Chebyshev sweep k from `source`; 1: residual of `source`; 2: Jacobi sweep from `source`
 template <int OP, bool H>
-__global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
-  if (!mg_enabled(s, phase)) return;
-  __shared__ int sh_off[3 * 125], sh_delta[125];
-  stage_stencil(m, l, sh_off, sh_delta);
-  const MGLevel& L = m.lv[l];
-  RowCtx rc = row_ctx(L.lanes, L.count);
+__device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
   float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
   if (!rc.valid || rc.lane != 0) return;
   int i = rc.row;
@@ -101,6 +96,15 @@ __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int so
   if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = xs + m.vec[at(m, l, 4, i)] * r; return; }
   float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = xs + d;
+}
+template <int OP, bool H>
+__global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  __shared__ int sh_off[3 * 125], sh_delta[125];
+  stage_stencil(m, l, sh_off, sh_delta);
+  const MGLevel& L = m.lv[l];
+  RowCtx rc = row_ctx(L.lanes, L.count);
+  coarse_row<OP, H>(m, l, source, k, rc, sh_off, sh_delta);
 }
 // op 8 into level l (from l-1) with G lanes per coarse node over the fine support
 __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
@@ -160,52 +164,16 @@ __global__ void __launch_bounds__(256) k_mg_residual(Ctx s, MGDev m, int l, int 
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   m.vec[at(m, l, 6, i)] = m.vec[at(m, l, 0, i)] - mapply0<H>(m, field, (int)i);
 }
-// op 8: exact P^T residual gather into level l from level l-1
-__global__ void __launch_bounds__(256) k_mg_restrict(Ctx s, MGDev m, int l, int phase) {
-  if (!mg_enabled(s, phase)) return;
-  int64_t i = GID(); if (i >= m.lv[l].count) return;
-  int parent = l - 1; const MGLevel& P = m.lv[parent];
-  I3 nf = mdims(m, parent), nl = mdims(m, l), qc = coord(i, nl);
-  float sum = 0;
-  if (P.aggregate) {
-    if (mactive(m, l, (int)i))
-      for (int z = 0; z < P.f[2]; z++) for (int y = 0; y < P.f[1]; y++) for (int x = 0; x < P.f[0]; x++) {
-        int j = mwrap(m, parent, mk(qc.x * P.f[0] + x, qc.y * P.f[1] + y, qc.z * P.f[2] + z));
-        if (mactive(m, parent, j)) sum += m.vec[at(m, parent, 6, j)];
-      }
-  } else {
-    int lo[3] = {qc.x * P.f[0], qc.y * P.f[1], qc.z * P.f[2]}, ext[3] = {1, 1, 1}, nfa[3] = {nf.x, nf.y, nf.z}, nla[3] = {nl.x, nl.y, nl.z}, qca[3] = {qc.x, qc.y, qc.z};
-    for (int axis = 0; axis < 3; axis++) if (P.f[axis] > 1) {
-      lo[axis]--; ext[axis] = 4;
-      if ((axis == 2 || (axis == 0 && m.perx)) && nfa[axis] <= 4) { lo[axis] = 0; ext[axis] = nfa[axis]; }
-    }
-    if (mactive(m, l, (int)i))
-      for (int z = 0; z < ext[2]; z++) for (int y = 0; y < ext[1]; y++) for (int x = 0; x < ext[0]; x++) {
-        int j = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)); if (!mactive(m, parent, j)) continue;
-        I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z}; float weight = 1;
-        for (int axis = 0; axis < 3; axis++) {
-          float pos = ((float)qa[axis] + .5f) / (float)P.f[axis] - .5f; int base = (int)floorf(pos); float t = fract(pos);
-          float w = 0;
-          for (int bit = 0; bit < P.f[axis]; bit++) {
-            int nb = base + bit;
-            if (axis == 2 || (axis == 0 && m.perx)) nb = pmod(nb, nla[axis]);
-            else if (axis == 0) nb = iclamp(nb, 0, nla[axis] - 1); else nb = nb > 0 ? nb : 0;
-            if (nb == qca[axis]) w += P.f[axis] == 1 ? 1 : (bit == 1 ? t : 1 - t);
-          }
-          weight *= w;
-        }
-        sum += weight * m.vec[at(m, parent, 6, j)];
-      }
-  }
-  m.vec[at(m, l, 0, i)] = sum; m.vec[at(m, l, 1, i)] = 0; m.vec[at(m, l, 2, i)] = 0;
-}
 // op 9: x_l += P x_{l+1}
-__global__ void __launch_bounds__(256) k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
-  if (!mg_enabled(s, phase)) return;
-  int64_t i = GID(); if (i >= m.lv[l].count) return;
+__device__ __forceinline__ void prolong_row(const MGDev& m, int l, int ff, int cf, int64_t i) {
   float add = 0;
   if (mactive(m, l, (int)i)) { int ids[8]; float ws[8]; int size = mtransfers(m, l, (int)i, ids, ws); for (int k = 0; k < size; k++) add += ws[k] * m.vec[at(m, l + 1, cf, ids[k])]; }
   m.vec[at(m, l, ff, i)] += add;
+}
+__global__ void __launch_bounds__(256) k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  int64_t i = GID(); if (i >= m.lv[l].count) return;
+  prolong_row(m, l, ff, cf, i);
 }
 // pressure_mg_coarse.glslinc:50-53 (op 12): z = L^-T (L^-1 b), one group
 __global__ void __launch_bounds__(256) k_mg_coarse_solve(Ctx s, MGDev m, int l, int phase) {
