@@ -242,6 +242,8 @@ __device__ __forceinline__ float4 group_reduce(float4 v) {
 template <bool MAXW>
 __device__ __forceinline__ float4 global_reduce(const float4* partial, int64_t groups) {
   int lid = threadIdx.x; float4 v = make_float4(0, 0, 0, 0);
+  // unrolled: the loads run ahead, each lane still adds its partials in index order
+#pragma unroll 8
   for (int64_t k = lid; k < groups; k += WV_RED) {
     float4 p = partial[k];
     v.x += p.x; v.y += p.y; v.z += p.z; v.w = MAXW ? gmax(v.w, p.w) : v.w + p.w;

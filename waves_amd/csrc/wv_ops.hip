@@ -142,12 +142,20 @@ __global__ void __launch_bounds__(256) k_inject(Ctx s) { s.c[19] = 1; }  // op 1
 // LDS and lane 0 adds them in index order.
 template <typename F>
 __device__ __forceinline__ void serial_partials(const float4* partial, int64_t groups, F&& fold) {
-  __shared__ float4 stage[512];
-  for (int64_t base = 0; base < groups; base += 512) {
+  // in index order on one lane (the reference's lane-0 loop); the partials are
+  // staged through LDS in large chunks and the fold is unrolled so the LDS
+  // reads run ahead of the dependent additions
+  constexpr int CHUNK = 2048;
+  __shared__ float4 stage[CHUNK];
+  for (int64_t base = 0; base < groups; base += CHUNK) {
     __syncthreads();
-    for (int k = threadIdx.x; k < 512 && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
+    for (int k = threadIdx.x; k < CHUNK && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
     __syncthreads();
-    if (threadIdx.x == 0) { int64_t n = groups - base < 512 ? groups - base : 512; for (int k = 0; k < n; k++) fold(stage[k]); }
+    if (threadIdx.x == 0) {
+      int n = (int)(groups - base < CHUNK ? groups - base : CHUNK);
+#pragma unroll 16
+      for (int k = 0; k < n; k++) fold(stage[k]);
+    }
   }
   __syncthreads();
 }
@@ -165,6 +173,7 @@ __global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int final) {
   const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // coupled_motion.glslinc:362-369: a max is order-free, so reduce it in parallel
   float m = 0; 
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) m = gmax(m, s.partial[k].y);
   float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
   if (threadIdx.x == 0) {
@@ -267,6 +276,7 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
   const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // coupled_motion.glslinc:402-408: min/max in parallel, the two sums in index order
   float mn = 1e20f, mx = -1e20f;
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
   __shared__ float zs, ws;
@@ -406,6 +416,7 @@ __global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, int final) {
   }
   const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   float mn = 1, mx = 0;
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
   if (threadIdx.x == 0) { s.c[88] = b.x; s.c[89] = b.y; if (b.x < 0 || b.y > 1) { s.c[7] = 23; s.c[3]++; } }
@@ -636,6 +647,7 @@ __global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, int final) {
   }
   const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   float m = 0;
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) m = gmax(m, s.partial[k].y);
   float4 r = group_reduce<RED_DIAG>(make_float4(m, 0, 0, 0));
   if (threadIdx.x == 0) s.c[27] = r.x;
@@ -683,6 +695,7 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int final) {
   const int64_t G = (s.N + WV_RED - 1) / WV_RED;
   // pressure_pass.glslinc:78-103: maxima in parallel, the volume in index order
   float4 mx = make_float4(0, 0, 0, 0);
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
   mx = group_reduce<RED_DIAG>(mx);
   __shared__ float vol;

@@ -36,6 +36,7 @@ __device__ __forceinline__ d4 group_reduce_d(d4 v) {
 }
 __device__ __forceinline__ d4 global_reduce_d(const double4* partial, int64_t groups) {
   d4 v = mkd(0, 0, 0, 0);
+#pragma unroll 8
   for (int64_t k = threadIdx.x; k < groups; k += WV_RED) { double4 p = partial[k]; v.x += p.x; v.y += p.y; v.z += p.z; v.w = v.w < p.w ? p.w : v.w; }
   return group_reduce_d(v);
 }
