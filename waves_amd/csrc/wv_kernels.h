@@ -17,6 +17,8 @@ struct MGLevel {
   int margin[3];   // largest |offset| per axis of the stencil: rows this far from every edge are interior
   int lanes;       // lanes cooperating on one row in the coarse-level kernels (power of two <= 32)
   int box[3];      // extent of one fine row's (A P) box over coarse nodes (Galerkin stage 1)
+  int wrap1;       // the stencil reaches at most one period along z (and x when periodic):
+                   // a neighbour index is row + delta with one +-period correction
 };
 struct MGDev {
   int L, perx;

@@ -62,7 +62,30 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
     const float* v = m.vec + L.vec_off + (int64_t)field * N;
     // unrolled so several coefficient and neighbour loads are in flight; the
     // additions stay in k order
-    if (interior) {
+    if (L.wrap1) {
+      // every row on one path: row + delta, one period added or removed where
+      // the offset crosses the periodic z (x) edge.  Entries towards the
+      // non-periodic edges hold a zero coefficient (no coarse node there), so
+      // their index only has to stay inside the level.
+      const int nz = L.n[2], nx = L.n[0];
+      auto entry = [&](int k) {
+        int j = rc.row + sh_delta[k];
+        int zz = q.z + sh_off[3 * k + 2];
+        j += zz < 0 ? N : (zz >= nz ? -N : 0);
+        if (m.perx) { int xx = q.x + sh_off[3 * k]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
+        j = min(max(j, 0), N - 1);
+        return (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[j];
+      };
+      if (rc.G == 1) {
+        // one row per thread (the large coarse levels): a unit stride lets the
+        // compiler keep more of the row's loads in flight
+#pragma unroll 13
+        for (int k = 0; k < W; k++) sum += entry(k);
+      } else {
+#pragma unroll 8
+        for (int k = rc.lane; k < W; k += rc.G) sum += entry(k);
+      }
+    } else if (interior) {
 #pragma unroll 8
       for (int k = rc.lane; k < W; k += rc.G) sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[rc.row + sh_delta[k]];
     } else {

@@ -114,6 +114,7 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
       if (std::abs(o.x) <= 2 && std::abs(o.y) <= 2 && std::abs(o.z) <= 2) p.slots[125 * l + (o.x + 2 + 5 * (o.y + 2 + 5 * (o.z + 2)))] = (signed char)k;
       pos += 3;
     }
+    L.wrap1 = L.margin[2] <= L.n[2] && (!perx || L.margin[0] <= L.n[0]);
   }
   // Galerkin stage-1 boxes: coarse span of the parents of a fine row's stencil
   p.ap_floats = 0;
@@ -136,10 +137,12 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     if (e > 64) return false;  // stage-1 LDS holds 64 box entries per fine row
     p.ap_floats = std::max(p.ap_floats, ((e + 3) / 4 * 4) * F.count);  // rows padded to 4 floats
   }
-  // lanes per row: enough threads that a sweep's load chains overlap (about
-  // 2^20 threads, 16 waves per SIMD), also on the large coarse levels
+  // lanes per row: enough threads that a sweep's load chains overlap on the
+  // small levels (about 2^18 threads); a level with more rows than that keeps
+  // one row per thread, whose unit-stride loop issues its loads back to back
+  // (measured faster than 4 lanes per row on the 297,600-row level 1)
   const char* lt = getenv("WAVES_MG_LANE_TARGET");
-  const int64_t lane_target = lt ? std::atoll(lt) : (1 << 20);
+  const int64_t lane_target = lt ? std::atoll(lt) : (1 << 18);
   for (int l = 0; l < p.L; l++) {
     MGLevel& L = p.lv[l]; int g = 1;
     while (g < 32 && (int64_t)L.count * g < lane_target && g * 2 <= L.width) g *= 2;
