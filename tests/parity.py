@@ -102,3 +102,43 @@ def make_pair(settings):
 def start_pair(o: O.Oracle, g: S.Solver):
     o.initial()
     g.start()
+
+
+def coarse_edge_entries(o: O.Oracle):
+    """Per coarse level l >= 1: (first row entry, count, width, mask[width, count])
+    marking the row entries whose stencil offset leaves the level through a
+    non-periodic face, where no coarse node exists.  The offsets are read off a
+    central row of the oracle's (value, column) rows; every entry with a nonzero
+    coefficient is checked to point at the wrapped target those offsets give."""
+    rows = o.get(O.FIELD_MG_ROWS).reshape(-1, 2)
+    cols, vals = rows[:, 1].view(np.int32), rows[:, 0]
+    perx = bool(o.settings.periodic_x)
+    out = []
+    for l, meta in enumerate(o.levels()):
+        if l == 0:
+            continue
+        nx, ny, nz = meta[:3]
+        N, W, r0 = meta[8], meta[11], meta[12]
+        q0 = np.array([nx // 2, ny // 2, nz // 2])
+        i0 = int(q0[0] + nx * (q0[1] + ny * q0[2]))
+        i = np.arange(N)
+        q = np.stack([i % nx, (i // nx) % ny, i // (nx * ny)], 1)
+        mask = np.zeros((W, N), bool)
+        for k in range(W):
+            c = int(cols[r0 + k * N + i0])
+            d = np.array([c % nx, (c // nx) % ny, c // (nx * ny)]) - q0
+            d[2] = (d[2] + nz // 2) % nz - nz // 2
+            if perx:
+                d[0] = (d[0] + nx // 2) % nx - nx // 2
+            t = q + d
+            t[:, 2] %= nz
+            if perx:
+                t[:, 0] %= nx
+            out_k = (t[:, 0] < 0) | (t[:, 0] >= nx) | (t[:, 1] < 0) | (t[:, 1] >= ny)
+            target = t[:, 0] + nx * (t[:, 1] + ny * t[:, 2])
+            e = r0 + k * N + i
+            live = ~out_k & (vals[e] != 0)
+            assert np.array_equal(cols[e][live], target[live]), ("stencil offsets not recovered", l, k)
+            mask[k] = out_k
+        out.append((r0, N, W, mask))
+    return out

@@ -52,6 +52,10 @@ def mg_report(o, g, tol=1e-5):
     rel = float(np.abs(ro - rg).max() / max(np.abs(ro).max(), 1e-30))
     say("  coarse rows rel diff", rel)
     assert rel < tol
+    # the coarse sweeps address every row by delta + one periodic correction,
+    # which needs the entries past a non-periodic edge to be exactly zero
+    for r0, N, W, mask in P.coarse_edge_entries(o):
+        assert np.all(rg[r0: r0 + W * N].reshape(W, N)[mask] == 0), ("edge entries", r0)
 
 
 def _batch_report(s, quota=1.0, groups=None, check=True, mg_tol=1e-5):

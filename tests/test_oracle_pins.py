@@ -313,6 +313,25 @@ def test_galerkin_matches_dense_triple_product():
         A = np.where(np.outer(a, a), Cl, 0.0)
 
 
+@pytest.mark.parametrize("periodic_x", [False, True])
+def test_galerkin_entries_past_an_edge_are_zero(periodic_x):
+    """A coarse row's stencil entry that points past a non-periodic face (where
+    there is no coarse node) is exactly zero: P has no column there, so P^T A P
+    has no entry.  The device sweeps rely on it to address every row by a
+    linear delta with a single periodic correction."""
+    from tests import parity as P
+    s = small_flume(amplitude=0.06, n=(32, 48, 8), periodic_x=periodic_x)
+    o = mg_pair_after_build(s)
+    rows = o.get(O.FIELD_MG_ROWS).reshape(-1, 2)[:, 0]
+    edges = P.coarse_edge_entries(o)
+    assert len(edges) >= 2
+    for r0, N, W, mask in edges:
+        vals = rows[r0: r0 + W * N].reshape(W, N)
+        assert mask.any()
+        assert np.all(vals[mask] == 0)
+        assert np.any(vals[~mask] != 0)
+
+
 def test_coarse_cholesky_inverse_matches_numpy():
     s = small_flume(amplitude=0.06)
     o = mg_pair_after_build(s)
