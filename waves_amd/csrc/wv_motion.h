@@ -181,11 +181,15 @@ __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis,
 // velocity at x (or the cached sv), at mid = x - 0.5 h u(x), and every component
 // at end = x - h u(mid), `component`'s from the forward field if forward_end.
 // A stage whose three samples are all direct issues their 48 loads together;
-// otherwise its samples run one at a time through the general path.
-struct Rk2Trace { F3 sv, um, mid, end; float value, l0, l1; bool vs, vm, ve; };
+// otherwise its samples run one at a time through the general path.  With
+// DEFER the trace stops at the first stage that would need the general path
+// and reports `deferred` (the caller queues the face for a second pass that
+// traces it again in full, so only those faces' waves run the general code).
+struct Rk2Trace { F3 sv, um, mid, end; float value, l0, l1; bool vs, vm, ve, deferred; };
+template <bool DEFER = false>
 __device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int component, bool forward_end, bool have_sv, F3 sv) {
   Rk2Trace t;
-  t.sv = sv; t.um = mkf(0, 0, 0); t.end = x; t.value = 0; t.l0 = 0; t.l1 = 0; t.vs = true; t.vm = true; t.ve = true;
+  t.sv = sv; t.um = mkf(0, 0, 0); t.end = x; t.value = 0; t.l0 = 0; t.l1 = 0; t.vs = true; t.vm = true; t.ve = true; t.deferred = false;
   t.mid = have_sv ? mkf(x.x - .5f * h * sv.x, x.y - .5f * h * sv.y, x.z - .5f * h * sv.z) : x;
 #pragma unroll 1
   for (int stage = have_sv ? 1 : 0; stage < 3; stage++) {
@@ -194,6 +198,7 @@ __device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int
          f2 = stage == 2 && component == 2 && forward_end;
     SampleAt a0 = sample_at(s, p, 0), a1 = sample_at(s, p, 1), a2 = sample_at(s, p, 2);
     float v0, v1, v2, lo0, lo1, lo2, hi0, hi1, hi2; bool ok0, ok1, ok2;
+    if (DEFER && !(a0.direct && a1.direct && a2.direct)) { t.deferred = true; return t; }
     if (a0.direct && a1.direct && a2.direct) {
       v0 = sample_direct(s, a0, 0, f0, ok0, lo0, hi0);
       v1 = sample_direct(s, a1, 1, f1, ok1, lo1, hi1);

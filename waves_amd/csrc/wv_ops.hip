@@ -296,10 +296,11 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
 // coupled_motion.glslinc:168-188, 409-426 (transport-cache and plain forms)
 // CACHE mirrors numerics flag 4 (set for the whole run): one kernel per form
 // keeps each variant's registers and code to its own path
-template <bool CORRECT, bool CACHE>
-__global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
-  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  int64_t i = GID(); if (i >= s.NF) return;
+// One face of the MacCormack pass (coupled_motion.glslinc:142-188).  With
+// DEFER (the cached form's first pass) a face whose trace would need the
+// general sampling path writes nothing and returns false.
+template <bool CORRECT, bool CACHE, bool DEFER>
+__device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i) {
   I3 q; int axis; unpack_face(s, i, q, axis);
   F3 x = mkf(((float)q.x + .5f) * s.h[0], ((float)q.y + .5f) * s.h[1], ((float)q.z + .5f) * s.h[2]);
   addc(x, axis, -(.5f * s.h[axis]));
@@ -309,7 +310,8 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
     if (CACHE) {
       int64_t base = trace_base(s), nf = s.NF;
       if (!CORRECT) {
-        Rk2Trace t = rk2_samples(s, x, dt, axis, false, false, mkf(0, 0, 0));
+        Rk2Trace t = rk2_samples<DEFER>(s, x, dt, axis, false, false, mkf(0, 0, 0));
+        if (DEFER && t.deferred) return false;
         value = t.value;
         s.motion[base + i] = t.l0; s.motion[base + nf + i] = t.l1;
         s.motion[base + 2 * nf + i] = t.vs && t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) ? 1 : 0;
@@ -319,7 +321,8 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
         if (s.motion[base + 2 * nf + i] < .5f) value = forward;
         else {
           F3 sv = mkf(s.motion[base + 3 * nf + i], s.motion[base + 4 * nf + i], s.motion[base + 5 * nf + i]);
-          Rk2Trace t = rk2_samples(s, x, -dt, axis, true, true, sv);
+          Rk2Trace t = rk2_samples<DEFER>(s, x, -dt, axis, true, true, sv);
+          if (DEFER && t.deferred) return false;
           float corrected = forward + .5f * (getv(s, acc, U_(s) + i) - t.value);
           value = t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) && corrected >= s.motion[base + i] && corrected <= s.motion[base + nf + i] ? corrected : forward;
         }
@@ -335,7 +338,43 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
     }
   }
   if (!CORRECT) s.motion[forward_base(s) + i] = value; else putv(s, candidate(s), U_(s) + i, value);
+  return true;
 }
+// the uncached form in one pass
+template <bool CORRECT>
+__global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID(); if (i >= s.NF) return;
+  maccormack_face<CORRECT, false, false>(s, i);
+}
+// The cached form in two passes.  About 8 % of the traced faces sample next to
+// the bed, the ocean ghost or the periodic z edge (the general path); spread
+// over the grid they put the general code into almost half of the waves.  The
+// first pass traces every face whose samples are all direct reads and queues
+// the others (one atomic per wave); the second traces the queued faces in
+// dense waves.  Each face is computed by the same code either way.
+__global__ void k_maccormack_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
+template <bool CORRECT>
+__global__ void __launch_bounds__(256) k_maccormack_direct(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  int64_t i = GID();
+  bool defer = i < s.NF && !maccormack_face<CORRECT, true, true>(s, i);
+  unsigned long long ballot = __ballot(defer);
+  if (ballot) {
+    int lane = __lane_id(), leader = __ffsll((long long)ballot) - 1;
+    int base = 0;
+    if (lane == leader) base = (int)atomicAdd(&s.mq_count[CORRECT ? 1 : 0], (unsigned)__popcll(ballot));
+    base = __shfl(base, leader);
+    if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
+  }
+}
+template <bool CORRECT>
+__global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s) {
+  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+  const int64_t n = s.mq_count[CORRECT ? 1 : 0], stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = GID(); j < n; j += stride) maccormack_face<CORRECT, true, false>(s, s.mq[j]);
+}
+constexpr int MAC_DEFERRED_BLOCKS = 1024;
 
 // --------------------------------------------------- coastal relaxation -----
 __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
@@ -764,8 +803,15 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 23: L256(k_reconstruct<1>, s.N); break;
   case 24: L256(k_flux, s.NF); break;
   case 25: LRED(k_vof_update, 4); break;
-  case 27: if (s.flags & 4) L256((k_maccormack<false, true>), s.NF); else L256((k_maccormack<false, false>), s.NF); break;
-  case 28: if (s.flags & 4) L256((k_maccormack<true, true>), s.NF); else L256((k_maccormack<true, false>), s.NF); break;
+  case 27: case 28: {
+    const bool corr = op == 28;
+    if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }
+    hipLaunchKernelGGL(k_maccormack_queue_reset, dim3(1), dim3(64), 0, st, s, corr ? 1 : 0);
+    if (corr) L256(k_maccormack_direct<true>, s.NF); else L256(k_maccormack_direct<false>, s.NF);
+    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(MAC_DEFERRED_BLOCKS), dim3(256), 0, st, s);
+    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(MAC_DEFERRED_BLOCKS), dim3(256), 0, st, s);
+    break;
+  }
   case 30: L256(k_ext_seed, s.NF); break;
   case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 256)), dim3(256), 0, st, s, (op - 31) % 2); break;
   case 34: L256(k_ext_fill, s.NF); break;

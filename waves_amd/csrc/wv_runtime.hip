@@ -263,6 +263,8 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.dctl, 16);
   ALLOC(c.pacc, c.N);
   ALLOC(c.aux, 2 * c.N);
+  ALLOC(c.mq, c.NF);
+  ALLOC(c.mq_count, 4);
   ALLOC(s->ctl_timing, WV_NCTRL);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
