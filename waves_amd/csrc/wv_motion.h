@@ -173,6 +173,28 @@ __device__ __forceinline__ float sample_general(const Ctx& s, const SampleAt& at
   float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
   return fmaf(t.z, upper - lower, lower);
 }
+// The general sample with eight lanes per face (the deferred MacCormack
+// faces): lane `sub` of the group evaluates corner `sub` through the same
+// donor code, the eight values and supports are gathered by 8-wide shuffles,
+// and every lane then runs the corner order and the fma interpolation of the
+// single-lane form, so all lanes hold the same, identical result.
+__device__ __forceinline__ float sample_general_coop(const Ctx& s, const SampleAt& at, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi, int sub) {
+  I3 fn = dims(s); addc(fn, axis, 1);
+  int acc = accepted(s);
+  bool mine_s;
+  float mine = sample_tap(s, at.lo, sub & 1, (sub >> 1) & 1, sub >> 2, axis, forward, at.direct, fn, acc, mine_s);
+  float v[8]; bool sup[8];
+#pragma unroll
+  for (int c8 = 0; c8 < 8; c8++) { v[c8] = __shfl(mine, c8, 8); sup[c8] = __shfl((int)mine_s, c8, 8) != 0; }
+  lim_lo = 1e20f; lim_hi = -1e20f; valid = true;
+#pragma unroll
+  for (int c8 = 0; c8 < 8; c8++) corner_update(at.t, c8, v[c8], sup[c8], valid, lim_lo, lim_hi);
+  F3 t = at.t;
+  float a = fmaf(t.x, v[1] - v[0], v[0]), b = fmaf(t.x, v[3] - v[2], v[2]);
+  float d = fmaf(t.x, v[5] - v[4], v[4]), e = fmaf(t.x, v[7] - v[6], v[6]);
+  float lower = fmaf(t.y, b - a, a), upper = fmaf(t.y, e - d, d);
+  return fmaf(t.z, upper - lower, lower);
+}
 __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
   SampleAt at = sample_at(s, x, axis);
   return sample_general(s, at, axis, forward, valid, lim_lo, lim_hi);
@@ -186,8 +208,8 @@ __device__ __forceinline__ float sampled_component(const Ctx& s, F3 x, int axis,
 // and reports `deferred` (the caller queues the face for a second pass that
 // traces it again in full, so only those faces' waves run the general code).
 struct Rk2Trace { F3 sv, um, mid, end; float value, l0, l1; bool vs, vm, ve, deferred; };
-template <bool DEFER = false>
-__device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int component, bool forward_end, bool have_sv, F3 sv) {
+template <bool DEFER = false, bool COOP = false>
+__device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int component, bool forward_end, bool have_sv, F3 sv, int sub = 0) {
   Rk2Trace t;
   t.sv = sv; t.um = mkf(0, 0, 0); t.end = x; t.value = 0; t.l0 = 0; t.l1 = 0; t.vs = true; t.vm = true; t.ve = true; t.deferred = false;
   t.mid = have_sv ? mkf(x.x - .5f * h * sv.x, x.y - .5f * h * sv.y, x.z - .5f * h * sv.z) : x;
@@ -203,6 +225,10 @@ __device__ __forceinline__ Rk2Trace rk2_samples(const Ctx& s, F3 x, float h, int
       v0 = sample_direct(s, a0, 0, f0, ok0, lo0, hi0);
       v1 = sample_direct(s, a1, 1, f1, ok1, lo1, hi1);
       v2 = sample_direct(s, a2, 2, f2, ok2, lo2, hi2);
+    } else if (COOP) {
+      v0 = sample_general_coop(s, a0, 0, f0, ok0, lo0, hi0, sub);
+      v1 = sample_general_coop(s, a1, 1, f1, ok1, lo1, hi1, sub);
+      v2 = sample_general_coop(s, a2, 2, f2, ok2, lo2, hi2, sub);
     } else {
       v0 = sample_general(s, a0, 0, f0, ok0, lo0, hi0);
       v1 = sample_general(s, a1, 1, f1, ok1, lo1, hi1);

@@ -299,8 +299,8 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
 // One face of the MacCormack pass (coupled_motion.glslinc:142-188).  With
 // DEFER (the cached form's first pass) a face whose trace would need the
 // general sampling path writes nothing and returns false.
-template <bool CORRECT, bool CACHE, bool DEFER>
-__device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i) {
+template <bool CORRECT, bool CACHE, bool DEFER, bool COOP = false>
+__device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i, int sub = 0) {
   I3 q; int axis; unpack_face(s, i, q, axis);
   F3 x = mkf(((float)q.x + .5f) * s.h[0], ((float)q.y + .5f) * s.h[1], ((float)q.z + .5f) * s.h[2]);
   addc(x, axis, -(.5f * s.h[axis]));
@@ -310,9 +310,10 @@ __device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i) {
     if (CACHE) {
       int64_t base = trace_base(s), nf = s.NF;
       if (!CORRECT) {
-        Rk2Trace t = rk2_samples<DEFER>(s, x, dt, axis, false, false, mkf(0, 0, 0));
+        Rk2Trace t = rk2_samples<DEFER, COOP>(s, x, dt, axis, false, false, mkf(0, 0, 0), sub);
         if (DEFER && t.deferred) return false;
         value = t.value;
+        if (COOP && sub != 0) return true;  // lane 0 of the group writes
         s.motion[base + i] = t.l0; s.motion[base + nf + i] = t.l1;
         s.motion[base + 2 * nf + i] = t.vs && t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) ? 1 : 0;
         s.motion[base + 3 * nf + i] = t.sv.x; s.motion[base + 4 * nf + i] = t.sv.y; s.motion[base + 5 * nf + i] = t.sv.z;
@@ -321,7 +322,7 @@ __device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i) {
         if (s.motion[base + 2 * nf + i] < .5f) value = forward;
         else {
           F3 sv = mkf(s.motion[base + 3 * nf + i], s.motion[base + 4 * nf + i], s.motion[base + 5 * nf + i]);
-          Rk2Trace t = rk2_samples<DEFER>(s, x, -dt, axis, true, true, sv);
+          Rk2Trace t = rk2_samples<DEFER, COOP>(s, x, -dt, axis, true, true, sv, sub);
           if (DEFER && t.deferred) return false;
           float corrected = forward + .5f * (getv(s, acc, U_(s) + i) - t.value);
           value = t.vm && t.ve && trace_points_clear(s, x, t.mid, t.end) && corrected >= s.motion[base + i] && corrected <= s.motion[base + nf + i] ? corrected : forward;
@@ -337,6 +338,7 @@ __device__ __forceinline__ bool maccormack_face(const Ctx& s, int64_t i) {
       value = valid && vf && trace_supported(s, x, dt) && trace_supported(s, x, -dt) && corrected >= d0 && corrected <= d1 ? corrected : forward;
     }
   }
+  if (COOP && sub != 0) return true;
   if (!CORRECT) s.motion[forward_base(s) + i] = value; else putv(s, candidate(s), U_(s) + i, value);
   return true;
 }
@@ -368,13 +370,16 @@ __global__ void __launch_bounds__(256) k_maccormack_direct(Ctx s) {
     if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
   }
 }
+// eight lanes per queued face: the deferred traces are few and latency-bound,
+// and the corners of a general sample are independent donor evaluations
 template <bool CORRECT>
 __global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  const int64_t n = s.mq_count[CORRECT ? 1 : 0], stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t j = GID(); j < n; j += stride) maccormack_face<CORRECT, true, false>(s, s.mq[j]);
+  const int64_t n = s.mq_count[CORRECT ? 1 : 0], groups = (int64_t)gridDim.x * blockDim.x / 8;
+  const int sub = (int)(threadIdx.x & 7);
+  for (int64_t j = GID() / 8; j < n; j += groups) maccormack_face<CORRECT, true, false, true>(s, s.mq[j], sub);
 }
-constexpr int MAC_DEFERRED_BLOCKS = 1024;
+constexpr int MAC_DEFERRED_BLOCKS = 2048;
 
 // --------------------------------------------------- coastal relaxation -----
 __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
