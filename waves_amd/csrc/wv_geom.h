@@ -244,8 +244,11 @@ __device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 
   return best;
 }
 // pressure_geometry.glslinc:157-182: exact signed distance at a cell centre
-__device__ __forceinline__ float geometry_distance(const Ctx& s, I3 q, int gbank) {
+// The value of a cell that needs no search (an interface cell, or one whose
+// six neighbours all lie on its own side), else `seed` is set.
+__device__ __forceinline__ float distance_head(const Ctx& s, I3 q, int gbank, bool& seed) {
   float f = geometry_fraction(s, q, gbank); F3 h = mkf(s.h[0], s.h[1], s.h[2]);
+  seed = false;
   if (f > 0 && f < 1) {
     float4 e = s.plane[index3(q, dims(s))];
     return (dot3(mkf(e.x, e.y, e.z), mkf(.5f, .5f, .5f)) - e.w) / len3(mkf(e.x / h.x, e.y / h.y, e.z / h.z));
@@ -257,17 +260,29 @@ __device__ __forceinline__ float geometry_distance(const Ctx& s, I3 q, int gbank
     if (other > 0 && other < 1) { float4 e = s.plane[index3(nb, dims(s))]; inside = dot3(mkf(e.x, e.y, e.z), mkf(.5f, .5f, .5f)) < e.w; }
     if (inside != (f >= 1)) needs_seed = true;
   }
-  if (!needs_seed) return (f >= 1 ? -1 : 1) * len3(h) * 4;
-  float best = len3(h) * 4;
-  for (int x = -1; x <= 1; x++) for (int y = -1; y <= 1; y++) for (int z = -1; z <= 1; z++) {
-    I3 nbr = mk(q.x + x, q.y + y, q.z + z), wr = wrap_cell(s, nbr);
-    if (outside(s, wr) || solid(s, wr)) continue;
-    float fraction = geometry_fraction(s, wr, gbank);
-    F3 point = mkf(((float)(q.x - nbr.x) + .5f) * h.x, ((float)(q.y - nbr.y) + .5f) * h.y, ((float)(q.z - nbr.z) + .5f) * h.z);
-    F3 d = mkf(point.x - gclamp(point.x, 0, h.x), point.y - gclamp(point.y, 0, h.y), point.z - gclamp(point.z, 0, h.z));
-    if (len3(d) >= best) continue;
-    if (fraction > 0 && fraction < 1) best = gmin(best, polygon_distance(s, point, s.plane[index3(wr, dims(s))]));
-    for (int axis = 0; axis < 3; axis++) best = gmin(best, transition_distance(s, point, nbr, axis, gbank));
-  }
-  return (f >= 1 ? -1 : 1) * best;
+  seed = needs_seed;
+  return (f >= 1 ? -1 : 1) * len3(h) * 4;
 }
+// Neighbour k (x-major over the 3x3x3 block, the search order) of a seeded
+// cell: the distance to its box and the least of its polygon and transition
+// distances; +inf where the neighbour is outside or solid.
+__device__ __forceinline__ void distance_neighbour(const Ctx& s, I3 q, int k, int gbank, float& box, float& m) {
+  const float inf = __int_as_float(0x7f800000);
+  box = inf; m = inf;
+  I3 nbr = mk(q.x + k / 9 - 1, q.y + (k / 3) % 3 - 1, q.z + k % 3 - 1), wr = wrap_cell(s, nbr);
+  if (outside(s, wr) || solid(s, wr)) return;
+  F3 h = mkf(s.h[0], s.h[1], s.h[2]);
+  float fraction = geometry_fraction(s, wr, gbank);
+  F3 point = mkf(((float)(q.x - nbr.x) + .5f) * h.x, ((float)(q.y - nbr.y) + .5f) * h.y, ((float)(q.z - nbr.z) + .5f) * h.z);
+  F3 d = mkf(point.x - gclamp(point.x, 0, h.x), point.y - gclamp(point.y, 0, h.y), point.z - gclamp(point.z, 0, h.z));
+  box = len3(d);
+  float c = inf;
+  if (fraction > 0 && fraction < 1) c = gmin(c, polygon_distance(s, point, s.plane[index3(wr, dims(s))]));
+  for (int axis = 0; axis < 3; axis++) c = gmin(c, transition_distance(s, point, nbr, axis, gbank));
+  m = c;
+}
+// pressure_geometry.glslinc:157-182.  The search keeps a running best and
+// skips a neighbour whose box is no nearer; min(best, m_k) equals the
+// reference's chain of mins over the neighbour's candidates (none is NaN-
+// sticky: gmin keeps the left operand when the right one is NaN).
+__device__ __forceinline__ float distance_step(float best, float box, float m) { return box >= best ? best : gmin(best, m); }

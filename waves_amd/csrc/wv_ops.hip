@@ -97,12 +97,56 @@ __global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
   } else s.plane[i] = e;
 }
 // pressure_pass.glslinc:131-137 (op 18)
-__global__ void __launch_bounds__(256) k_distance(Ctx s) {
-  int64_t i = GID(); if (i >= s.N) return;
-  if (s.c[4] > .5f && s.c[5] < .5f) return;
-  I3 q = coord(i, dims(s));
-  float ph = geometry_distance(s, q, geometry_bank(s, false));
+// Work queues (Ctx::mq): a first pass handles the cells or faces on a cheap
+// path and appends the others, one atomic per wave; a second pass works the
+// queue in dense waves.  Slots: 0/1 MacCormack ops 27/28, 2 distance.
+__device__ __forceinline__ void queue_append(const Ctx& s, bool defer, int64_t i, int slot) {
+  unsigned long long ballot = __ballot(defer);
+  if (!ballot) return;
+  int lane = __lane_id(), leader = __ffsll((long long)ballot) - 1;
+  int base = 0;
+  if (lane == leader) base = (int)atomicAdd(&s.mq_count[slot], (unsigned)__popcll(ballot));
+  base = __shfl(base, leader);
+  if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
+}
+__global__ void k_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
+constexpr int QUEUE_BLOCKS = 2048;
+// The exact distance in two passes: interface cells and cells with no
+// opposite-side neighbour (almost all) directly; the seeded cells along the
+// interface, each a 27-neighbour polygon search, with 32 lanes per cell.
+__device__ __forceinline__ void write_distance(const Ctx& s, int64_t i, I3 q, float ph) {
   s.geo[i] = make_float2(ph, solid(s, q) ? -1.0f : (ph < 0 ? 1.0f : 0.0f));
+}
+__global__ void __launch_bounds__(256) k_distance_direct(Ctx s) {
+  if (s.c[4] > .5f && s.c[5] < .5f) return;
+  int64_t i = GID(); bool defer = false;
+  if (i < s.N) {
+    I3 q = coord(i, dims(s)); bool seed;
+    float ph = distance_head(s, q, geometry_bank(s, false), seed);
+    if (seed) defer = true; else write_distance(s, i, q, ph);
+  }
+  queue_append(s, defer, i, 2);
+}
+// lane k < 27 of a 32-lane group evaluates neighbour k; lane 0 gathers the 27
+// (box, candidate) pairs and runs the reference's pruned search in order
+__global__ void __launch_bounds__(256) k_distance_deferred(Ctx s) {
+  if (s.c[4] > .5f && s.c[5] < .5f) return;
+  const int64_t n = s.mq_count[2], groups = (int64_t)gridDim.x * blockDim.x / 32;
+  const int sub = (int)(threadIdx.x & 31);
+  const int gb = geometry_bank(s, false);
+  for (int64_t j = GID() / 32; j < n; j += groups) {
+    int64_t i = s.mq[j]; I3 q = coord(i, dims(s));
+    float box = __int_as_float(0x7f800000), m = box;
+    if (sub < 27) distance_neighbour(s, q, sub, gb, box, m);
+    bool seed; float head = distance_head(s, q, gb, seed);
+    float best = fabsf(head);  // 4 |h|, the search's start
+    // every lane runs the same in-order search over the gathered pairs
+    for (int k = 0; k < 27; k++) {
+      float bk = __shfl(box, k, 32), mk = __shfl(m, k, 32);
+      best = distance_step(best, bk, mk);
+    }
+    if (sub == 0) write_distance(s, i, q, (geometry_fraction(s, q, gb) >= 1 ? -1 : 1) * best);
+  }
 }
 // pressure_pass.glslinc:108-115 (op 16)
 __global__ void __launch_bounds__(256) k_coefficients(Ctx s) {
@@ -137,27 +181,39 @@ __global__ void __launch_bounds__(256) k_trial_start(Ctx s) {
 }
 __global__ void __launch_bounds__(256) k_inject(Ctx s) { s.c[19] = 1; }  // op 15, fault injection
 
-// Serial in-order sum of the block partials' component k (the reference's lane-0
-// loops, e.g. pressure_pass.glslinc:80): the whole group stages the partials in
-// LDS and lane 0 adds them in index order.
-template <typename F>
-__device__ __forceinline__ void serial_partials(const float4* partial, int64_t groups, F&& fold) {
-  // in index order on one lane (the reference's lane-0 loop); the partials are
-  // staged through LDS in large chunks and the fold is unrolled so the LDS
-  // reads run ahead of the dependent additions
-  constexpr int CHUNK = 2048;
+// Serial in-order sums of the block partials (the reference's lane-0 loops,
+// e.g. pressure_pass.glslinc:80, coupled_motion.glslinc:402-408): the group
+// stages the partials in LDS in large chunks; lane 0 adds component CA and
+// lane 1 component CB (CB < 0: none) in index order, so the two dependent
+// chains advance in the same instructions, and the LDS reads of a batch are
+// issued ahead of its additions.  Lane 0 returns (sum CA, sum CB).
+template <int CA, int CB>
+__device__ __forceinline__ float2 serial_sums(const float4* partial, int64_t groups) {
+  constexpr int CHUNK = 2048, BATCH = 16;
   __shared__ float4 stage[CHUNK];
+  const int lane = threadIdx.x;
+  const int comp = lane == 0 ? CA : CB;
+  float acc = 0;
   for (int64_t base = 0; base < groups; base += CHUNK) {
     __syncthreads();
-    for (int k = threadIdx.x; k < CHUNK && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
+    for (int k = lane; k < CHUNK && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int n = (int)(groups - base < CHUNK ? groups - base : CHUNK);
-#pragma unroll 16
-      for (int k = 0; k < n; k++) fold(stage[k]);
+    if (lane < (CB >= 0 ? 2 : 1)) {
+      const float* src = reinterpret_cast<const float*>(stage) + comp;
+      int n = (int)(groups - base < CHUNK ? groups - base : CHUNK), k = 0;
+      for (; k + BATCH <= n; k += BATCH) {
+        float v[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; b++) v[b] = src[4 * (k + b)];
+#pragma unroll
+        for (int b = 0; b < BATCH; b++) acc += v[b];
+      }
+      for (; k < n; k++) acc += src[4 * k];
     }
   }
+  float other = __shfl(acc, 1, 64);
   __syncthreads();
+  return make_float2(acc, other);
 }
 
 // ---------------------------------------------------- CFL / dt (20 + 21) -----
@@ -279,9 +335,8 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
 #pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
-  __shared__ float zs, ws;
-  if (threadIdx.x == 0) { zs = 0; ws = 0; }
-  serial_partials(s.partial, G, [&](float4 p) { zs += p.z; ws += p.w; });
+  const float2 zw = serial_sums<2, 3>(s.partial, G);
+  const float zs = zw.x, ws = zw.y;
   if (threadIdx.x == 0) {
     float* c = s.c; float cell = s.h[0] * s.h[1] * s.h[2];
     uint32_t step = (uint32_t)(int64_t)c[32] - 1u, sw = (uint32_t)c[21];
@@ -355,20 +410,12 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
 // first pass traces every face whose samples are all direct reads and queues
 // the others (one atomic per wave); the second traces the queued faces in
 // dense waves.  Each face is computed by the same code either way.
-__global__ void k_maccormack_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
 template <bool CORRECT>
 __global__ void __launch_bounds__(256) k_maccormack_direct(Ctx s) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID();
   bool defer = i < s.NF && !maccormack_face<CORRECT, true, true>(s, i);
-  unsigned long long ballot = __ballot(defer);
-  if (ballot) {
-    int lane = __lane_id(), leader = __ffsll((long long)ballot) - 1;
-    int base = 0;
-    if (lane == leader) base = (int)atomicAdd(&s.mq_count[CORRECT ? 1 : 0], (unsigned)__popcll(ballot));
-    base = __shfl(base, leader);
-    if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
-  }
+  queue_append(s, defer, i, CORRECT ? 1 : 0);
 }
 // eight lanes per queued face: the deferred traces are few and latency-bound,
 // and the corners of a general sample are independent donor evaluations
@@ -379,7 +426,6 @@ __global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s) {
   const int sub = (int)(threadIdx.x & 7);
   for (int64_t j = GID() / 8; j < n; j += groups) maccormack_face<CORRECT, true, false, true>(s, s.mq[j], sub);
 }
-constexpr int MAC_DEFERRED_BLOCKS = 2048;
 
 // --------------------------------------------------- coastal relaxation -----
 __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
@@ -742,9 +788,7 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int final) {
 #pragma unroll 8
   for (int64_t k = threadIdx.x; k < G; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
   mx = group_reduce<RED_DIAG>(mx);
-  __shared__ float vol;
-  if (threadIdx.x == 0) vol = 0;
-  serial_partials(s.partial, G, [&](float4 p) { vol += p.w; });
+  const float vol = serial_sums<3, -1>(s.partial, G).x;
   if (threadIdx.x != 0) return;
   float* c = s.c; float4 v = make_float4(mx.x, mx.y, mx.z, vol); float dt = timestep(s);
   c[14] = v.x; c[15] = v.y; c[16] = v.z; c[17] = v.w;
@@ -801,7 +845,11 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 14: L256(k_export_plain, s.N); break;
   case 16: L256(k_coefficients, s.N); break;
   case 17: L256(k_reconstruct<0>, s.N); break;
-  case 18: L256(k_distance, s.N); break;
+  case 18:
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
+    L256(k_distance_direct, s.N);
+    hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
+    break;
   case 19: L256(k_reconstruct<2>, s.N); break;
   case 20: LRED(k_cfl, 3); break;
   case 22: L256(k_bank_copy, s.stride); break;
@@ -811,10 +859,10 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 27: case 28: {
     const bool corr = op == 28;
     if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }
-    hipLaunchKernelGGL(k_maccormack_queue_reset, dim3(1), dim3(64), 0, st, s, corr ? 1 : 0);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, corr ? 1 : 0);
     if (corr) L256(k_maccormack_direct<true>, s.NF); else L256(k_maccormack_direct<false>, s.NF);
-    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(MAC_DEFERRED_BLOCKS), dim3(256), 0, st, s);
-    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(MAC_DEFERRED_BLOCKS), dim3(256), 0, st, s);
+    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
+    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
     break;
   }
   case 30: L256(k_ext_seed, s.NF); break;
