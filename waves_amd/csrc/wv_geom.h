@@ -107,22 +107,40 @@ __device__ __forceinline__ float hf_fraction(const Ctx& s, I3 q, int source, int
 }
 
 // pressure_geometry.glslinc:72-90: weighted Youngs gradient, one-sided next to solids
+// Every cell the stencil touches lies in the 3x3x3 block around q: its solid
+// flags and fractions are read first, 27 independent loads issued together,
+// and the loops below (fully unrolled, constant block indices) run on
+// registers.  The values and the order of the arithmetic are the reference's.
 __device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source, int gbank) {
+  bool sb[27]; float fb[27];
+#pragma unroll
+  for (int c = 0; c < 27; c++) {
+    I3 p = mk(q.x + c % 3 - 1, q.y + (c / 3) % 3 - 1, q.z + c / 9 - 1);
+    sb[c] = solid(s, p); fb[c] = hf_fraction(s, p, source, gbank);
+  }
   F3 g = mkf(0, 0, 0);
+#pragma unroll
   for (int axis = 0; axis < 3; axis++) {
-    int b = (axis + 1) % 3, d = (axis + 2) % 3, weights = 0;
+    const int b = (axis + 1) % 3, d = (axis + 2) % 3;
+    int weights = 0;
     float acc = 0;
-    for (int j = -1; j <= 1; j++) for (int k = -1; k <= 1; k++) {
-      I3 middle = q; addc(middle, b, j); addc(middle, d, k); if (solid(s, middle)) continue;
-      I3 l = middle, r = middle; addc(l, axis, -1); addc(r, axis, 1);
-      bool left = !solid(s, l), right = !solid(s, r); if (!left && !right) continue;
-      float difference;
-      if (left && right) difference = hf_fraction(s, l, source, gbank) - hf_fraction(s, r, source, gbank);
-      else if (left) difference = 2 * (hf_fraction(s, l, source, gbank) - hf_fraction(s, middle, source, gbank));
-      else difference = 2 * (hf_fraction(s, middle, source, gbank) - hf_fraction(s, r, source, gbank));
-      int weight = (j == 0 ? 2 : 1) * (k == 0 ? 2 : 1); weights += weight;
-      acc += (float)weight * difference / (32 * s.h[axis]);
-    }
+#pragma unroll
+    for (int j = -1; j <= 1; j++)
+#pragma unroll
+      for (int k = -1; k <= 1; k++) {
+        int o[3] = {0, 0, 0}; o[b] = j; o[d] = k;
+        const int cm = (o[0] + 1) + 3 * ((o[1] + 1) + 3 * (o[2] + 1));
+        const int step = axis == 0 ? 1 : (axis == 1 ? 3 : 9);
+        const int cl = cm - step, cr = cm + step;
+        if (sb[cm]) continue;
+        bool left = !sb[cl], right = !sb[cr]; if (!left && !right) continue;
+        float difference;
+        if (left && right) difference = fb[cl] - fb[cr];
+        else if (left) difference = 2 * (fb[cl] - fb[cm]);
+        else difference = 2 * (fb[cm] - fb[cr]);
+        int weight = (j == 0 ? 2 : 1) * (k == 0 ? 2 : 1); weights += weight;
+        acc += (float)weight * difference / (32 * s.h[axis]);
+      }
     if (weights > 0) acc *= 16.0f / (float)weights;
     setc(g, axis, acc);
   }
@@ -158,13 +176,17 @@ __device__ __forceinline__ F3 height_normal(const Ctx& s, I3 q, F3 youngs, int s
   return r;
 }
 // pressure_geometry.glslinc:91-98 / coupled_motion.glslinc:50-56
-__device__ __forceinline__ float4 plane_from_gradient(const Ctx& s, I3 q, F3 g, int source, int gbank, float f) {
+__device__ __forceinline__ F3 normal_from_gradient(const Ctx& s, I3 q, F3 g, int source, int gbank) {
   if (s.flags & 1) g = height_normal(s, q, g, source, gbank);
   if (len3(g) < 1e-9f) g = mkf(0, 1, 0);
   float L = len3(g);
   F3 n = mkf(g.x / L * s.h[0], g.y / L * s.h[1], g.z / L * s.h[2]);
   float l1 = dot3(mkf(fabsf(n.x), fabsf(n.y), fabsf(n.z)), mkf(1, 1, 1));
   n.x /= l1; n.y /= l1; n.z /= l1;
+  return n;
+}
+__device__ __forceinline__ float4 plane_from_gradient(const Ctx& s, I3 q, F3 g, int source, int gbank, float f) {
+  F3 n = normal_from_gradient(s, q, g, source, gbank);
   return make_float4(n.x, n.y, n.z, inverse_volume(n, f));
 }
 

@@ -78,25 +78,6 @@ __global__ void __launch_bounds__(256) k_perturb(Ctx s) {
 }
 
 // ------------------------------------------------------- geometry (17,18,16) --
-// MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
-template <int MODE>
-__global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
-  int64_t i = GID(); if (i >= s.N) return;
-  if (MODE == 0 && s.c[4] > .5f && s.c[5] < .5f) return;
-  if (MODE == 1 && !(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (MODE == 2 && s.c[4] > .5f) return;
-  I3 q = coord(i, dims(s));
-  int gbank = geometry_bank(s, MODE == 2);
-  float f = MODE == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
-  float4 e;
-  if (f <= 0 || f >= 1) e = make_float4(0, 1, 0, f);
-  else e = plane_from_gradient(s, q, interface_gradient(s, q, MODE == 1 ? 1 : 0, gbank), MODE == 1 ? 1 : 0, gbank, f);
-  if (MODE == 2) {
-    float fa = getv(s, accepted(s), i);
-    s.img_planes[i] = e; s.img_kinds[i] = fa <= 0 ? 0 : (fa >= 1 ? 1 : 2);
-  } else s.plane[i] = e;
-}
-// pressure_pass.glslinc:131-137 (op 18)
 // Work queues (Ctx::mq): a first pass handles the cells or faces on a cheap
 // path and appends the others, one atomic per wave; a second pass works the
 // queue in dense waves.  Slots: 0/1 MacCormack ops 27/28, 2 distance.
@@ -111,6 +92,31 @@ __device__ __forceinline__ void queue_append(const Ctx& s, bool defer, int64_t i
 }
 __global__ void k_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
 constexpr int QUEUE_BLOCKS = 2048;
+// MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
+template <int MODE>
+__device__ __forceinline__ bool reconstruct_gate(const Ctx& s) {
+  if (MODE == 0) return !(s.c[4] > .5f && s.c[5] < .5f);
+  if (MODE == 1) return s.c[5] > .5f && s.c[7] < .5f;
+  return !(s.c[4] > .5f);
+}
+template <int MODE>
+__device__ __forceinline__ void reconstruct_store(const Ctx& s, int64_t i, float4 e) {
+  if (MODE == 2) s.img_planes[i] = e; else s.plane[i] = e;
+}
+// pressure_pass.glslinc:116-130 (ops 17, 19) and coupled_motion.glslinc:50-56 (op 23):
+// the PLIC plane of every cell (interface cells: Youngs gradient + bisection)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
+  if (!reconstruct_gate<MODE>(s)) return;
+  int64_t i = GID(); if (i >= s.N) return;
+  I3 q = coord(i, dims(s));
+  const int gbank = geometry_bank(s, MODE == 2), source = MODE == 1 ? 1 : 0;
+  float f = MODE == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+  float4 e = (f <= 0 || f >= 1) ? make_float4(0, 1, 0, f) : plane_from_gradient(s, q, interface_gradient(s, q, source, gbank), source, gbank, f);
+  reconstruct_store<MODE>(s, i, e);
+  if (MODE == 2) { float fa = getv(s, accepted(s), i); s.img_kinds[i] = fa <= 0 ? 0 : (fa >= 1 ? 1 : 2); }
+}
+// pressure_pass.glslinc:131-137 (op 18)
 // The exact distance in two passes: interface cells and cells with no
 // opposite-side neighbour (almost all) directly; the seeded cells along the
 // interface, each a 27-neighbour polygon search, with 32 lanes per cell.
