@@ -86,6 +86,13 @@ __device__ __forceinline__ int64_t P_(const Ctx& s) { return s.N + s.NU + s.NV +
 __device__ __forceinline__ int64_t index3(I3 q, I3 n) { return (int64_t)q.x + (int64_t)n.x * (q.y + (int64_t)n.y * q.z); }
 __device__ __forceinline__ I3 dims(const Ctx& s) { return mk(s.nx, s.ny, s.nz); }
 __device__ __forceinline__ I3 coord(int64_t i, I3 n) {
+  // 32-bit unsigned division (about a fifth of the 64-bit sequence) whenever
+  // the index and a plane of the grid fit, which is every grid here
+  if ((uint64_t)i < 0x100000000ull && (uint64_t)n.x * (uint64_t)n.y < 0x100000000ull) {
+    uint32_t u = (uint32_t)i, xy = (uint32_t)n.x * (uint32_t)n.y;
+    uint32_t z = u / xy, r = u - z * xy, y = r / (uint32_t)n.x;
+    return mk((int)(r - y * (uint32_t)n.x), (int)y, (int)z);
+  }
   int64_t xy = (int64_t)n.x * n.y;
   int z = (int)(i / xy); int64_t r = i - (int64_t)z * xy; int y = (int)(r / n.x);
   return mk((int)(r - (int64_t)y * n.x), y, z);

@@ -42,7 +42,9 @@ __global__ void __launch_bounds__(256) k_mg_cheb(Ctx s, MGDev m, int l, int sour
 // through the periodic/clamped wrap of pressure_multigrid.glslinc:36-41.
 struct RowCtx { int row, lane, G; bool valid; };
 __device__ __forceinline__ RowCtx row_ctx(int G, int N) {
-  RowCtx r; int64_t gid = GID(); r.G = G; r.row = (int)(gid / G); r.lane = (int)(gid % G); r.valid = r.row < N; return r;
+  // G is a power of two: shift and mask instead of a 64-bit division
+  RowCtx r; int64_t gid = GID(); int lg = __ffs(G) - 1;
+  r.G = G; r.row = (int)(gid >> lg); r.lane = (int)(gid & (G - 1)); r.valid = r.row < N; return r;
 }
 __device__ __forceinline__ float group_sum(float v, int G) {
   for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
