@@ -15,7 +15,8 @@ struct MGLevel {
   int64_t vec_off; // first float of this level's 8 vector fields
   int64_t row_off; // first float of this level's k-major rows (levels >= 1)
   int margin[3];   // largest |offset| per axis of the stencil: rows this far from every edge are interior
-  int lanes;       // lanes cooperating on one row in the coarse-level kernels (power of two <= 32)
+  int lanes;       // lanes cooperating on one row in the coarse-level sweeps (power of two <= 32)
+  int rlanes;      // lanes cooperating on one node in the restriction into this level (power of two <= 32)
   int box[3];      // extent of one fine row's (A P) box over coarse nodes (Galerkin stage 1)
   int wrap1;       // the stencil reaches at most one period along z (and x when periodic):
                    // a neighbour index is row + delta with one +-period correction

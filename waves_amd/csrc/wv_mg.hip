@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int so
 __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
   if (!mg_enabled(s, phase)) return;
   const MGLevel& C = m.lv[l]; int parent = l - 1; const MGLevel& P = m.lv[parent];
-  RowCtx rc = row_ctx(C.lanes, C.count);
+  RowCtx rc = row_ctx(C.rlanes, C.count);
   float sum = 0;
   I3 nf = mdims(m, parent), nl = mdims(m, l);
   if (rc.valid && mactive(m, l, rc.row)) {
@@ -628,7 +628,7 @@ struct CycleBuilder {
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
     else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
-    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(coop_blocks(l + 1)), dim3(256), 0, st, s, m, l + 1, phase);
+    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
     return smooth(l, field);

@@ -147,6 +147,13 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     MGLevel& L = p.lv[l]; int g = 1;
     while (g < 32 && (int64_t)L.count * g < lane_target && g * 2 <= L.width) g *= 2;
     L.lanes = l == 0 ? 1 : g;
+    // the restriction gathers a 4x4x4 (trilinear) or 2x2x2 box per node with
+    // no unrolled stencil: it keeps about 2^20 lanes on every level
+    int box = 1;
+    if (l > 0) for (int a = 0; a < 3; a++) box *= p.lv[l - 1].aggregate ? p.lv[l - 1].f[a] : (p.lv[l - 1].f[a] > 1 ? 4 : 1);
+    int r = 1;
+    while (r < 32 && 2 * r <= box && (int64_t)L.count * r < (1 << 20)) r *= 2;
+    L.rlanes = r;
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
   p.node_count = node_count; p.row_count = row_count;
