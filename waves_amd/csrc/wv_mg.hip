@@ -466,9 +466,11 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
   if (!build_on(s, m, false)) return;
   extern __shared__ float acc[];  // width(l+1) * GAL_THREADS, one slice per lane
   int lc = l + 1; const MGLevel& C = m.lv[lc]; const MGLevel& F = m.lv[l];
-  int G = C.lanes > 8 ? 8 : C.lanes;
+  // lanes per coarse node over its fine support: the restriction's split, at
+  // most 8 (lane 0 folds the G accumulator slices)
+  const int G = C.rlanes > 8 ? 8 : C.rlanes;
   int64_t gid = GID(); int tid = threadIdx.x;
-  int64_t I = gid / G; int lane = (int)(gid % G);
+  int64_t I = gid >> (__ffs(G) - 1); int lane = (int)(gid & (G - 1));
   int W = C.width;
   for (int k = 0; k < W; k++) acc[k * GAL_THREADS + tid] = 0;
   const signed char* slots = m.slots + 125 * lc;
@@ -530,12 +532,16 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
 }
 // Keep the canonical (upper) entry of every coupling and mirror it, so the
 // coarse operator is exactly symmetric; non-canonical duplicate offsets stay 0.
+// SYM_LANES lanes per row, each taking every SYM_LANES-th entry: the entries
+// are independent (a mirrored entry reads a canonical one, never written here)
+#define SYM_LANES 8
 __global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
-  int64_t I = GID(); const MGLevel& C = m.lv[l]; if (I >= C.count) return;
+  int64_t gid = GID(); const MGLevel& C = m.lv[l];
+  int64_t I = gid / SYM_LANES; int lane = (int)(gid % SYM_LANES); if (I >= C.count) return;
   I3 q = coord(I, mdims(m, l)); const signed char* slots = m.slots + 125 * l;
   float* base = m.rows + C.row_off;
-  for (int k = 0; k < C.width; k++) {
+  for (int k = lane; k < C.width; k += SYM_LANES) {
     I3 off = moffset(m, l, k); int J = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
     if (J < 0) { base[(int64_t)k * C.count + I] = 0; continue; }
     I3 d = mdelta(m, l, (int)I, J);
@@ -591,10 +597,10 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
   hipLaunchKernelGGL(k_mg_l1_fine, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m);
   for (int l = 0; l + 1 < m.L; l++) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
-    int G = m.lv[l + 1].lanes > 8 ? 8 : m.lv[l + 1].lanes;
+    int G = m.lv[l + 1].rlanes > 8 ? 8 : m.lv[l + 1].rlanes;
     hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), 0, st, s, m, l);
     hipLaunchKernelGGL(k_mg_galerkin_ptap, dim3(nblocks((int64_t)m.lv[l + 1].count * G, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
-    hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
+    hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks((int64_t)m.lv[l + 1].count * SYM_LANES, 256)), dim3(256), 0, st, s, m, l + 1);
     hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
   }
   if (m.lv[m.L - 1].count <= 64) hipLaunchKernelGGL(k_mg_cholesky, dim3(1), dim3(64), 0, st, s, m, m.L - 1);
