@@ -530,6 +530,82 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
     row[(int64_t)k * C.count] = v;
   }
 }
+// Stage 2 as a gather per coarse entry: block b is coarse node I, lane k owns
+// the entry at stencil offset k and adds, over the fine support i of I (in
+// support order), w_iI times the box entries of (A P)_i whose coarse node is
+// J = I + offset k: one box position in the interior, an interval where the
+// transfer clamps (x without periodicity, y below 0), a period apart along a
+// periodic axis.  The entry counts only if offset k is J's minimal-image
+// offset (the slot the box entry maps to); other aliases of J stay 0 and are
+// zeroed by the symmetrisation.  No accumulator in LDS, a fixed order: the
+// operator is deterministic.
+#define GATHER_SUPPORT 64
+__global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int l) {
+  if (!build_on(s, m, false)) return;
+  const int lc = l + 1; const MGLevel& C = m.lv[lc]; const MGLevel& F = m.lv[l];
+  const int I = (int)blockIdx.x, k = threadIdx.x, W = C.width;
+  __shared__ float wsup[GATHER_SUPPORT];
+  __shared__ int isup[GATHER_SUPPORT], asup[GATHER_SUPPORT][3];
+  float* row = m.rows + C.row_off + I;
+  if (!mactive(m, lc, I)) {  // inactive rows are the identity
+    if (k < W) { I3 off = moffset(m, lc, k); row[(int64_t)k * C.count] = (off.x == 0 && off.y == 0 && off.z == 0) ? 1.0f : 0.0f; }
+    return;
+  }
+  const I3 qc = coord(I, mdims(m, lc)), nf = mdims(m, l), nc = mdims(m, lc);
+  int lo[3], hi[3]; const int qca[3] = {qc.x, qc.y, qc.z}, nfa[3] = {nf.x, nf.y, nf.z};
+  for (int a = 0; a < 3; a++) {
+    if (F.aggregate) { lo[a] = qca[a] * F.f[a]; hi[a] = lo[a] + F.f[a] - 1; if (hi[a] > nfa[a] - 1) hi[a] = nfa[a] - 1; }
+    else if (F.f[a] == 1) { lo[a] = qca[a]; hi[a] = qca[a]; }
+    else { lo[a] = 2 * qca[a] - 1; hi[a] = 2 * qca[a] + 2;
+      bool periodic = a == 2 || (a == 0 && m.perx);
+      if (!periodic) { if (lo[a] < 0) lo[a] = 0; if (hi[a] > nfa[a] - 1) hi[a] = nfa[a] - 1; }
+    }
+  }
+  const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, total = sx * sy * (hi[2] - lo[2] + 1);
+  if (k < total) {
+    int x = lo[0] + k % sx, y = lo[1] + (k / sx) % sy, z = lo[2] + k / (sx * sy);
+    int i = mwrap(m, l, mk(x, y, z)); float wI = 0;
+    if (i >= 0 && mactive(m, l, i)) {
+      int ids[8]; float ws[8]; int ni = mtransfers(m, l, i, ids, ws);
+      for (int a = 0; a < ni; a++) if (ids[a] == I) wI += ws[a];
+      I3 q = coord(i, nf); int qa[3] = {q.x, q.y, q.z};
+      for (int a = 0; a < 3; a++) asup[k][a] = parent_base(F, a, qa[a] - F.margin[a]);
+    }
+    wsup[k] = wI; isup[k] = i;
+  }
+  __syncthreads();
+  if (k >= W) return;
+  const I3 off = moffset(m, lc, k);
+  int J[3] = {qc.x + off.x, qc.y + off.y, qc.z + off.z};
+  bool inside = J[1] >= 0 && J[1] < nc.y;
+  if (m.perx) J[0] = pmod(J[0], nc.x); else inside = inside && J[0] >= 0 && J[0] < nc.x;
+  J[2] = pmod(J[2], nc.z);
+  float acc = 0;
+  if (inside) {
+    I3 d = mk(J[0] - qc.x, J[1] - qc.y, J[2] - qc.z);
+    if (m.perx) d.x = pmod(d.x + nc.x / 2, nc.x) - nc.x / 2;
+    d.z = pmod(d.z + nc.z / 2, nc.z) - nc.z / 2;
+    if (d.x == off.x && d.y == off.y && d.z == off.z) {
+      const int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2], Ep = (E0 * E1 * E2 + 3) & ~3;
+      for (int c = 0; c < total; c++) {
+        const float w = wsup[c]; if (w == 0) continue;
+        const int ax = asup[c][0], ay = asup[c][1], az = asup[c][2];
+        const float* T = m.ap + (int64_t)isup[c] * Ep;
+        // box ranges whose positions map to J along each axis
+        int ylo = J[1] == 0 ? 0 : J[1] - ay, yhi = J[1] - ay;
+        if (ylo < 0) ylo = 0; if (yhi > E1 - 1) yhi = E1 - 1;
+        int xlo, xhi, xstep;
+        if (m.perx) { xlo = pmod(J[0] - ax, nc.x); xhi = E0 - 1; xstep = nc.x; }
+        else { xlo = J[0] == 0 ? 0 : J[0] - ax; xhi = J[0] == nc.x - 1 ? E0 - 1 : J[0] - ax; xstep = 1;
+               if (xlo < 0) xlo = 0; if (xhi > E0 - 1) xhi = E0 - 1; }
+        for (int bz = pmod(J[2] - az, nc.z); bz < E2; bz += nc.z)
+          for (int by = ylo; by <= yhi; by++)
+            for (int bx = xlo; bx <= xhi; bx += xstep) acc += w * T[bx + E0 * (by + E1 * bz)];
+      }
+    }
+  }
+  row[(int64_t)k * C.count] = acc;
+}
 // Keep the canonical (upper) entry of every coupling and mirror it, so the
 // coarse operator is exactly symmetric; non-canonical duplicate offsets stay 0.
 // SYM_LANES lanes per row, each taking every SYM_LANES-th entry: the entries
@@ -599,7 +675,10 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
     int G = m.lv[l + 1].rlanes > 8 ? 8 : m.lv[l + 1].rlanes;
     hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), 0, st, s, m, l);
-    hipLaunchKernelGGL(k_mg_galerkin_ptap, dim3(nblocks((int64_t)m.lv[l + 1].count * G, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
+    // wide coarse stencils (125 entries: a block's 128 lanes) gather per entry;
+    // narrow ones split each node's fine support over lanes with LDS slices
+    if (m.lv[l + 1].width > 64) hipLaunchKernelGGL(k_mg_galerkin_gather, dim3(m.lv[l + 1].count), dim3(128), 0, st, s, m, l);
+    else hipLaunchKernelGGL(k_mg_galerkin_ptap, dim3(nblocks((int64_t)m.lv[l + 1].count * G, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
     hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks((int64_t)m.lv[l + 1].count * SYM_LANES, 256)), dim3(256), 0, st, s, m, l + 1);
     hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
   }
