@@ -686,6 +686,219 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
   hipLaunchKernelGGL(k_audit4, dim3(1), dim3(1), 0, st, s, m);
 }
 
+// ---- the smallest levels in one workgroup -----------------------------------
+// Below some level T the levels are a few hundred nodes: one launch per sweep
+// costs more than the sweep.  k_mg_tail runs cycle(T) -- the sweeps, residual
+// and restriction of each level on the way down, the dense coarse solve, the
+// prolongation and post-sweeps on the way up -- in one 1024-thread workgroup
+// with the levels' rows, vectors, stencil offsets and L^-1 in LDS, a barrier
+// between phases.  It mirrors the launch sequence phase for phase (one thread
+// per row, each row's sum in stencil order) and writes every vector field of
+// the tail levels back, so the level vectors end as the launches leave them
+// up to the rounding of the row sums' order.
+#define TAIL_THREADS 1024
+struct TailLayout { int T, L, coef[WV_MAXL], vec[WV_MAXL], off[WV_MAXL], linv, rhs, floats; };
+static TailLayout tail_layout(const MGDev& m, int T) {
+  TailLayout t; t.T = T; t.L = m.L; int at = 0;
+  for (int l = T; l < m.L; l++) {
+    const MGLevel& L = m.lv[l];
+    t.coef[l] = at; at += L.width * L.count;
+    t.vec[l] = at; at += 8 * L.count;
+    t.off[l] = at; at += 4 * L.width;  // dx, dy, dz, linear delta per entry
+  }
+  int nl = m.lv[m.L - 1].count;
+  t.linv = at; at += nl * nl; t.rhs = at; at += 64;
+  t.floats = at;
+  return t;
+}
+static int tail_lds_limit() {
+  static int limit = -1;
+  if (limit < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = v;
+  }
+  return limit;
+}
+// first tail level, or -1: the tail holds at least two levels, ends in the
+// dense coarse solve, and fits the workgroup's LDS
+static int tail_start(const MGDev& m) {
+  if (m.L < 3 || m.lv[m.L - 1].count > 64) return -1;
+  int best = -1;
+  for (int T = m.L - 2; T >= 1; T--) {
+    if ((size_t)tail_layout(m, T).floats * sizeof(float) > (size_t)tail_lds_limit()) break;
+    best = T;
+  }
+  return best;
+}
+template <bool H>
+__global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLayout tl, int phase, int smoothing) {
+  if (!mg_enabled(s, phase)) return;
+  extern __shared__ float sm[];
+  const int tid = threadIdx.x;
+  auto vec = [&](int l, int field) { return sm + tl.vec[l] + field * m.lv[l].count; };
+  auto active = [&](int l, int i) { return i >= 0 && vec(l, 3)[i] > .5f; };
+  for (int l = tl.T; l < tl.L; l++) {
+    const MGLevel& L = m.lv[l]; const int n = L.count, W = L.width;
+    for (int e = tid; e < W * n; e += TAIL_THREADS) sm[tl.coef[l] + e] = H ? bf16_to_f32(m.rows16[L.row_off + e]) : m.rows[L.row_off + e];
+    for (int e = tid; e < 8 * n; e += TAIL_THREADS) sm[tl.vec[l] + e] = m.vec[L.vec_off + e];
+    for (int k = tid; k < W; k += TAIL_THREADS) {
+      int* o = reinterpret_cast<int*>(sm) + tl.off[l] + 4 * k;
+      o[0] = m.offsets[3 * (L.off_start + k)]; o[1] = m.offsets[3 * (L.off_start + k) + 1];
+      o[2] = m.offsets[3 * (L.off_start + k) + 2]; o[3] = m.deltas[L.off_start + k];
+    }
+  }
+  const int nl = m.lv[tl.L - 1].count;
+  for (int e = tid; e < nl * nl; e += TAIL_THREADS) sm[tl.linv + e] = m.linv[e];
+  __syncthreads();
+  // A x_source at row i of level l, entries in stencil order (coarse_apply;
+  // on wrap1 levels the neighbour is row + delta with one periodic correction)
+  auto apply = [&](int l, int source, int i) {
+    const MGLevel& L = m.lv[l]; const int n = L.count, W = L.width;
+    const int* off = reinterpret_cast<const int*>(sm) + tl.off[l];
+    const float* A = sm + tl.coef[l]; const float* x = vec(l, source);
+    I3 q = coord(i, mdims(m, l)); float sum = 0;
+    if (L.wrap1) {
+      const int nz = L.n[2], nx = L.n[0];
+#pragma unroll 9
+      for (int k = 0; k < W; k++) {
+        int j = i + off[4 * k + 3], zz = q.z + off[4 * k + 2];
+        j += zz < 0 ? n : (zz >= nz ? -n : 0);
+        if (m.perx) { int xx = q.x + off[4 * k]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
+        j = min(max(j, 0), n - 1);
+        sum += A[k * n + i] * x[j];
+      }
+    } else {
+      for (int k = 0; k < W; k++) {
+        int j = mwrap(m, l, mk(q.x + off[4 * k], q.y + off[4 * k + 1], q.z + off[4 * k + 2]));
+        sum += A[k * n + i] * x[j >= 0 ? j : 0];
+      }
+    }
+    return sum;
+  };
+  // coarse_row OP 0 (Chebyshev sweep k) and OP 1 (residual)
+  auto sweep = [&](int l, int source, int k) {
+    const int n = m.lv[l].count;
+    for (int i = tid; i < n; i += TAIL_THREADS) {
+      float ax = apply(l, source, i), xs = vec(l, source)[i];
+      if (!active(l, i)) ax = xs;
+      float r = vec(l, 0)[i] - ax;
+      float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * vec(l, 7)[i]) + (float)(8 * k - 4) / (float)(2 * k + 1) * vec(l, 4)[i] * r;
+      vec(l, 7)[i] = d; vec(l, 3 - source)[i] = xs + d;
+    }
+    __syncthreads();
+  };
+  auto residual = [&](int l, int field) {
+    const int n = m.lv[l].count;
+    for (int i = tid; i < n; i += TAIL_THREADS) {
+      float ax = apply(l, field, i), xs = vec(l, field)[i];
+      if (!active(l, i)) ax = xs;
+      vec(l, 6)[i] = vec(l, 0)[i] - ax;
+    }
+    __syncthreads();
+  };
+  // k_mg_restrict_coop into level lc, children in order on one thread
+  auto restrict_to = [&](int lc) {
+    const MGLevel& C = m.lv[lc]; const int parent = lc - 1; const MGLevel& P = m.lv[parent];
+    I3 nf = mdims(m, parent), nlc = mdims(m, lc);
+    for (int I = tid; I < C.count; I += TAIL_THREADS) {
+      float sum = 0;
+      if (active(lc, I)) {
+        I3 qc = coord(I, nlc);
+        int lo[3], ext[3] = {1, 1, 1}, nfa[3] = {nf.x, nf.y, nf.z}, nla[3] = {nlc.x, nlc.y, nlc.z}, qca[3] = {qc.x, qc.y, qc.z};
+        for (int a = 0; a < 3; a++) lo[a] = qca[a] * P.f[a];
+        if (P.aggregate) { ext[0] = P.f[0]; ext[1] = P.f[1]; ext[2] = P.f[2]; }
+        else for (int a = 0; a < 3; a++) if (P.f[a] > 1) {
+          lo[a]--; ext[a] = 4;
+          if ((a == 2 || (a == 0 && m.perx)) && nfa[a] <= 4) { lo[a] = 0; ext[a] = nfa[a]; }
+        }
+        int total = ext[0] * ext[1] * ext[2];
+        for (int t = 0; t < total; t++) {
+          int x = t % ext[0], y = (t / ext[0]) % ext[1], z = t / (ext[0] * ext[1]);
+          int j = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)); if (!active(parent, j)) continue;
+          float weight = 1;
+          if (!P.aggregate) {
+            I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z};
+            for (int a = 0; a < 3; a++) {
+              float pos = ((float)qa[a] + .5f) / (float)P.f[a] - .5f; int base = (int)floorf(pos); float tt = fract(pos);
+              float w = 0;
+              for (int bit = 0; bit < P.f[a]; bit++) {
+                int nb = base + bit;
+                if (a == 2 || (a == 0 && m.perx)) nb = pmod(nb, nla[a]);
+                else if (a == 0) nb = iclamp(nb, 0, nla[a] - 1); else nb = nb > 0 ? nb : 0;
+                if (nb == qca[a]) w += P.f[a] == 1 ? 1 : (bit == 1 ? tt : 1 - tt);
+              }
+              weight *= w;
+            }
+          }
+          sum += weight * vec(parent, 6)[j];
+        }
+      }
+      vec(lc, 0)[I] = sum; vec(lc, 1)[I] = 0; vec(lc, 2)[I] = 0;
+    }
+    __syncthreads();
+  };
+  // prolong_row with the transfers of mtransfers, active flags from LDS
+  auto prolong = [&](int l, int ff, int cf) {
+    const MGLevel& L = m.lv[l]; I3 n1 = mdims(m, l + 1), nlv = mdims(m, l);
+    for (int i = tid; i < L.count; i += TAIL_THREADS) {
+      float add = 0;
+      if (active(l, i)) {
+        I3 q = coord(i, nlv);
+        if (L.aggregate) {
+          int j = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), n1);
+          if (active(l + 1, j)) add += 1 * vec(l + 1, cf)[j];
+        } else {
+          int base[3]; float t[3]; int qq[3] = {q.x, q.y, q.z};
+          for (int a = 0; a < 3; a++) { float pos = ((float)qq[a] + .5f) / (float)L.f[a] - .5f; base[a] = (int)floorf(pos); t[a] = fract(pos); }
+          for (int z = 0; z < L.f[2]; z++) for (int y = 0; y < L.f[1]; y++) for (int x = 0; x < L.f[0]; x++) {
+            int bit[3] = {x, y, z}; I3 nb = mk(base[0] + x, base[1] + y, base[2] + z); float ww = 1;
+            for (int a = 0; a < 3; a++) if (L.f[a] > 1) ww *= bit[a] == 1 ? t[a] : 1 - t[a];
+            if (nb.y >= n1.y) continue; if (nb.y < 0) nb.y = 0;
+            nb.x = m.perx ? pmod(nb.x, n1.x) : iclamp(nb.x, 0, n1.x - 1); nb.z = pmod(nb.z, n1.z);
+            int j = (int)index3(nb, n1); if (!active(l + 1, j) || ww == 0) continue;
+            add += ww * vec(l + 1, cf)[j];
+          }
+        }
+      }
+      vec(l, ff)[i] += add;
+    }
+    __syncthreads();
+  };
+  int fields[WV_MAXL];
+  for (int l = tl.T; l < tl.L - 1; l++) {
+    int f = 1;
+    for (int k = 1; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
+    fields[l] = f;
+    residual(l, f);
+    restrict_to(l + 1);
+  }
+  {  // k_mg_coarse_solve on the last level: x = L^-T (L^-1 b), field 1
+    const float* Li = sm + tl.linv; float* rhs = sm + tl.rhs; const int N = nl, lc = tl.L - 1;
+    if (tid < N) { float d = 0; for (int j = 0; j <= tid; j++) d += Li[tid * N + j] * vec(lc, 0)[j]; rhs[tid] = d; }
+    __syncthreads();
+    if (tid < N) { float d = 0; for (int j = tid; j < N; j++) d += Li[j * N + tid] * rhs[j]; vec(lc, 1)[tid] = d; }
+    __syncthreads();
+  }
+  int cf = 1;
+  for (int l = tl.L - 2; l >= tl.T; l--) {
+    int f = fields[l];
+    prolong(l, f, cf);
+    for (int k = 1; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
+    cf = f;
+  }
+  for (int l = tl.T; l < tl.L; l++) {
+    const MGLevel& L = m.lv[l];
+    for (int e = tid; e < 8 * L.count; e += TAIL_THREADS) m.vec[L.vec_off + e] = sm[tl.vec[l] + e];
+  }
+}
+// the field holding level T's result after the tail (the host's replay of the
+// field alternation above)
+static int tail_field(int smoothing) {
+  int down = smoothing % 2 ? 2 : 1;
+  return smoothing % 2 ? 3 - down : down;
+}
+
 template <bool H>
 struct CycleBuilder {
   const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
@@ -700,6 +913,11 @@ struct CycleBuilder {
   }
   // water/pressure_multigrid.gd:141-154
   int cycle(int l) {
+    if (l > 0 && l == tail_start(m)) {
+      TailLayout tl = tail_layout(m, l);
+      hipLaunchKernelGGL(k_mg_tail<H>, dim3(1), dim3(TAIL_THREADS), tl.floats * sizeof(float), st, s, m, tl, phase, smoothing);
+      return tail_field(smoothing);
+    }
     if (l == m.L - 1) {
       if (m.lv[l].count <= 64) { hipLaunchKernelGGL(k_mg_coarse_solve, dim3(1), dim3(64), 0, st, s, m, l, phase); return 1; }
       int tf = 1;
