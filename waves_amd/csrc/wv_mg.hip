@@ -109,11 +109,8 @@ __device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off
   __syncthreads();
 }
 // OP 0: Chebyshev sweep k from `source`; 1: residual of `source`; 2: Jacobi sweep from `source`
-template <int OP, bool H>
-__device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
-  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
-  if (!rc.valid || rc.lane != 0) return;
-  int i = rc.row;
+template <int OP>
+__device__ __forceinline__ void coarse_epilogue(const MGDev& m, int l, int source, int k, int i, float ax) {
   float xs = m.vec[at(m, l, source, i)];
   if (!mactive(m, l, i)) ax = xs;  // inactive rows are the identity (pressure_multigrid.glslinc:91)
   float r = m.vec[at(m, l, 0, i)] - ax;
@@ -121,6 +118,51 @@ __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, in
   if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = xs + m.vec[at(m, l, 4, i)] * r; return; }
   float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = xs + d;
+}
+template <int OP, bool H>
+__device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
+  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
+  if (!rc.valid || rc.lane != 0) return;
+  coarse_epilogue<OP>(m, l, source, k, rc.row, ax);
+}
+// Wide rows (125 entries): the four waves of a block share 64 consecutive
+// rows, wave w summing entries w, w + 4, ... with every load 64 rows wide
+// (256 contiguous bytes); the partials join in wave order in LDS.
+template <int OP, bool H>
+__global__ void __launch_bounds__(256) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
+  if (!mg_enabled(s, phase)) return;
+  __shared__ int sh_off[3 * 125], sh_delta[125];
+  __shared__ float part[4][64];
+  stage_stencil(m, l, sh_off, sh_delta);
+  const MGLevel& L = m.lv[l]; const int N = L.count, W = L.width;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = blockIdx.x * 64 + lane;
+  float sum = 0;
+  if (row < N) {
+    I3 q = coord(row, mdims(m, l));
+    const float* A = m.rows + L.row_off + row;
+    const uint16_t* A16 = m.rows16 + L.row_off + row;
+    const float* v = m.vec + L.vec_off + (int64_t)source * N;
+    if (L.wrap1) {
+      const int nz = L.n[2], nx = L.n[0];
+#pragma unroll 8
+      for (int e = wave; e < W; e += 4) {
+        int j = row + sh_delta[e], zz = q.z + sh_off[3 * e + 2];
+        j += zz < 0 ? N : (zz >= nz ? -N : 0);
+        if (m.perx) { int xx = q.x + sh_off[3 * e]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
+        j = min(max(j, 0), N - 1);
+        sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j];
+      }
+    } else {
+      for (int e = wave; e < W; e += 4) {
+        int j = mwrap(m, l, mk(q.x + sh_off[3 * e], q.y + sh_off[3 * e + 1], q.z + sh_off[3 * e + 2]));
+        sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j >= 0 ? j : 0];
+      }
+    }
+  }
+  part[wave][lane] = sum;
+  __syncthreads();
+  if (wave != 0 || row >= N) return;
+  coarse_epilogue<OP>(m, l, source, k, row, ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
 }
 template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
@@ -906,6 +948,7 @@ struct CycleBuilder {
   int smooth(int l, int field) {
     for (int sweep = 0; sweep < smoothing; sweep++) {
       if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
@@ -930,6 +973,7 @@ struct CycleBuilder {
     }
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
+    else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(256), 0, st, s, m, l, field, 0, phase);
     else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
     hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
