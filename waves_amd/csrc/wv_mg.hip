@@ -125,14 +125,15 @@ __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, in
   if (!rc.valid || rc.lane != 0) return;
   coarse_epilogue<OP>(m, l, source, k, rc.row, ax);
 }
-// Wide rows (125 entries): the four waves of a block share 64 consecutive
-// rows, wave w summing entries w, w + 4, ... with every load 64 rows wide
-// (256 contiguous bytes); the partials join in wave order in LDS.
+// Wide rows (125 entries): the WIDE_WAVES waves of a block share 64
+// consecutive rows, wave w summing entries w, w + WIDE_WAVES, ... with every
+// load 64 rows wide (256 contiguous bytes); the partials join in wave order.
+#define WIDE_WAVES 8
 template <int OP, bool H>
-__global__ void __launch_bounds__(256) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
+__global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ int sh_off[3 * 125], sh_delta[125];
-  __shared__ float part[4][64];
+  __shared__ float part[WIDE_WAVES][64];
   stage_stencil(m, l, sh_off, sh_delta);
   const MGLevel& L = m.lv[l]; const int N = L.count, W = L.width;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = blockIdx.x * 64 + lane;
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse_wide(Ctx s, MGDev m, int l, i
     if (L.wrap1) {
       const int nz = L.n[2], nx = L.n[0];
 #pragma unroll 8
-      for (int e = wave; e < W; e += 4) {
+      for (int e = wave; e < W; e += WIDE_WAVES) {
         int j = row + sh_delta[e], zz = q.z + sh_off[3 * e + 2];
         j += zz < 0 ? N : (zz >= nz ? -N : 0);
         if (m.perx) { int xx = q.x + sh_off[3 * e]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse_wide(Ctx s, MGDev m, int l, i
         sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j];
       }
     } else {
-      for (int e = wave; e < W; e += 4) {
+      for (int e = wave; e < W; e += WIDE_WAVES) {
         int j = mwrap(m, l, mk(q.x + sh_off[3 * e], q.y + sh_off[3 * e + 1], q.z + sh_off[3 * e + 2]));
         sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j >= 0 ? j : 0];
       }
@@ -162,7 +163,9 @@ __global__ void __launch_bounds__(256) k_mg_coarse_wide(Ctx s, MGDev m, int l, i
   part[wave][lane] = sum;
   __syncthreads();
   if (wave != 0 || row >= N) return;
-  coarse_epilogue<OP>(m, l, source, k, row, ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+  float ax = part[0][lane];
+  for (int w = 1; w < WIDE_WAVES; w++) ax += part[w][lane];
+  coarse_epilogue<OP>(m, l, source, k, row, ax);
 }
 template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
@@ -948,7 +951,7 @@ struct CycleBuilder {
   int smooth(int l, int field) {
     for (int sweep = 0; sweep < smoothing; sweep++) {
       if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
-      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(64 * WIDE_WAVES), 0, st, s, m, l, field, sweep + 1, phase);
       else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
@@ -973,7 +976,7 @@ struct CycleBuilder {
     }
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
-    else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(256), 0, st, s, m, l, field, 0, phase);
+    else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(64 * WIDE_WAVES), 0, st, s, m, l, field, 0, phase);
     else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
     hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
