@@ -125,11 +125,10 @@ __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, in
   if (!rc.valid || rc.lane != 0) return;
   coarse_epilogue<OP>(m, l, source, k, rc.row, ax);
 }
-// Wide rows (125 entries): the WIDE_WAVES waves of a block share 64
-// consecutive rows, wave w summing entries w, w + WIDE_WAVES, ... with every
-// load 64 rows wide (256 contiguous bytes); the partials join in wave order.
-#define WIDE_WAVES 8
-template <int OP, bool H>
+// Wide rows: the WIDE_WAVES waves of a block share 64 consecutive rows, wave
+// w summing entries w, w + WIDE_WAVES, ... with every load 64 rows wide (256
+// contiguous bytes); the partials join in wave order in LDS.
+template <int OP, bool H, int WIDE_WAVES>
 __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ int sh_off[3 * 125], sh_delta[125];
@@ -951,7 +950,7 @@ struct CycleBuilder {
   int smooth(int l, int field) {
     for (int sweep = 0; sweep < smoothing; sweep++) {
       if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
-      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(64 * WIDE_WAVES), 0, st, s, m, l, field, sweep + 1, phase);
+      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, sweep + 1, phase);
       else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
@@ -976,7 +975,7 @@ struct CycleBuilder {
     }
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
-    else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H>), dim3(nblocks(m.lv[l].count, 64)), dim3(64 * WIDE_WAVES), 0, st, s, m, l, field, 0, phase);
+    else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, 0, phase);
     else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
     hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), 0, st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
