@@ -214,7 +214,8 @@ def main():
     for key, kid, nbytes, label in (
             ("mg_coarse_cheb_level1", 4, count1 * (4 * width1 + 28),
              f"Chebyshev sweep on MG level 1 (k_mg_coarse<0>): {count1} rows x {width1}-point Galerkin stencil"),
-            ("maccormack_forward", 5, NF * 36, f"MacCormack forward trace (k_maccormack, op 27): {NF} faces"),
+            ("maccormack_forward", 5, NF * 36,
+             f"MacCormack forward trace (op 27: k_maccormack_direct + queued k_maccormack_deferred): {NF} faces"),
             ("mg_cheb_level0", 0, 52 * N, f"level-0 Chebyshev sweep (k_mg_cheb): {N} cells")):
         us = solver.time_kernel(kid, reps=200)
         achieved = nbytes / (us * 1e-6) / 1e9
