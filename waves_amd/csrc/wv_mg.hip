@@ -458,7 +458,7 @@ __device__ __forceinline__ int parent_node(const MGDev& m, int lc, const int P[3
 }
 __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
-  __shared__ float acc[64 * 128];
+  extern __shared__ float acc[];  // box entries x 128 lanes, sized per level
   const MGLevel& F = m.lv[l]; int lc = l + 1;
   int64_t i = GID(); int tid = threadIdx.x;
   int E0 = F.box[0], E1 = F.box[1], E = F.box[0] * F.box[1] * F.box[2];
@@ -718,7 +718,8 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
   for (int l = 0; l + 1 < m.L; l++) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
     int G = m.lv[l + 1].rlanes > 8 ? 8 : m.lv[l + 1].rlanes;
-    hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), 0, st, s, m, l);
+    const size_t ap_lds = sizeof(float) * 128 * m.lv[l].box[0] * m.lv[l].box[1] * m.lv[l].box[2];
+    hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), ap_lds, st, s, m, l);
     // wide coarse stencils (125 entries: a block's 128 lanes) gather per entry;
     // narrow ones split each node's fine support over lanes with LDS slices
     if (m.lv[l + 1].width > 64) hipLaunchKernelGGL(k_mg_galerkin_gather, dim3(m.lv[l + 1].count), dim3(128), 0, st, s, m, l);
