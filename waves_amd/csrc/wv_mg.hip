@@ -456,19 +456,23 @@ __device__ __forceinline__ int parent_node(const MGDev& m, int lc, const int P[3
   int z = pmod(P[2], n.z);
   return (int)index3(mk(x, y, z), n);
 }
-__global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l) {
+// G lanes per fine row (a power of two dividing 128): lane g takes entries
+// g, g + G, ... into its own slice of the accumulator, and lane 0 folds the
+// slices in lane order into (A P)_i.
+__global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l, int G) {
   if (!build_on(s, m, false)) return;
   extern __shared__ float acc[];  // box entries x 128 lanes, sized per level
   const MGLevel& F = m.lv[l]; int lc = l + 1;
-  int64_t i = GID(); int tid = threadIdx.x;
+  const int64_t gid = GID(); const int tid = threadIdx.x, lg = __ffs(G) - 1, lane = tid & (G - 1);
+  const int64_t i = gid >> lg;
   int E0 = F.box[0], E1 = F.box[1], E = F.box[0] * F.box[1] * F.box[2];
   for (int b = 0; b < E; b++) acc[b * 128 + tid] = 0;
-  if (i >= F.count) return;
-  if (!mactive(m, l, (int)i)) return;
+  const bool live = i < F.count && mactive(m, l, (int)i);
+  if (live) {
   I3 q = coord(i, mdims(m, l)); int qa[3] = {q.x, q.y, q.z};
   int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
   int width = F.width;
-  for (int k = 0; k < width; k++) {
+  for (int k = lane; k < width; k += G) {
     int off[3]; float aij; int j;
     if (l == 0) {
       off[0] = off[1] = off[2] = 0;
@@ -497,13 +501,15 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l) {
       acc[b * 128 + tid] += aij * w;
     }
   }
+  }
+  __syncthreads();
+  if (!live || lane != 0) return;
+  auto box = [&](int b) { if (b >= E) return 0.f; float v = acc[b * 128 + tid]; for (int g = 1; g < G; g++) v += acc[b * 128 + tid + g]; return v; };
   // (A P)_i is stored row-major, padded to a multiple of 4 floats, so that the
   // second stage reads a fine row's box with 16-byte loads
   const int Ep = (E + 3) & ~3;
   float4* T = (float4*)(m.ap + i * Ep);
-  for (int b = 0; b < Ep; b += 4)
-    T[b >> 2] = make_float4(acc[b * 128 + tid], b + 1 < E ? acc[(b + 1) * 128 + tid] : 0.f,
-                            b + 2 < E ? acc[(b + 2) * 128 + tid] : 0.f, b + 3 < E ? acc[(b + 3) * 128 + tid] : 0.f);
+  for (int b = 0; b < Ep; b += 4) T[b >> 2] = make_float4(box(b), box(b + 1), box(b + 2), box(b + 3));
 }
 #define GAL_THREADS 64
 __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m, int l) {
@@ -719,7 +725,9 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
     int G = m.lv[l + 1].rlanes > 8 ? 8 : m.lv[l + 1].rlanes;
     const size_t ap_lds = sizeof(float) * 128 * m.lv[l].box[0] * m.lv[l].box[1] * m.lv[l].box[2];
-    hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks(m.lv[l].count, 128)), dim3(128), ap_lds, st, s, m, l);
+    int ga = 1;  // lanes per fine row: about 2^18 lanes, at most 16 and the row width
+    while (ga < 16 && (int64_t)m.lv[l].count * ga < (1 << 18) && 2 * ga <= m.lv[l].width) ga *= 2;
+    hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks((int64_t)m.lv[l].count * ga, 128)), dim3(128), ap_lds, st, s, m, l, ga);
     // wide coarse stencils (125 entries: a block's 128 lanes) gather per entry;
     // narrow ones split each node's fine support over lanes with LDS slices
     if (m.lv[l + 1].width > 64) hipLaunchKernelGGL(k_mg_galerkin_gather, dim3(m.lv[l + 1].count), dim3(128), 0, st, s, m, l);
