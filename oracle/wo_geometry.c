@@ -26,7 +26,10 @@ float wo_cube_volume3(F3 normal, float alpha) {
   else if (t < y) result = (3 * t * t - 3 * t * x + x * x) / (6 * y * z);
   else if (x + y < z && t >= x + y) result = (t - 0.5f * (x + y)) / z;
   else {
-    float numerator = x * (3 * t * t - 3 * t * x + x * x) - powf(fmaxf_(t - y, 0.0f), 3.0f) - powf(fmaxf_(t - z, 0.0f), 3.0f);
+    /* GLSL pow(v, 3.0) carries only exp2/log2 precision; the cube is the
+       product (v * v) * v here and on the device */
+    float ty = fmaxf_(t - y, 0.0f), tz = fmaxf_(t - z, 0.0f);
+    float numerator = x * (3 * t * t - 3 * t * x + x * x) - ty * ty * ty - tz * tz * tz;
     result = numerator / (6 * x * y * z);
   }
   return complement ? 1.0f - result : result;

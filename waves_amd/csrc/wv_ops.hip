@@ -188,37 +188,67 @@ __global__ void __launch_bounds__(256) k_trial_start(Ctx s) {
 __global__ void __launch_bounds__(256) k_inject(Ctx s) { s.c[19] = 1; }  // op 15, fault injection
 
 // Serial in-order sums of the block partials (the reference's lane-0 loops,
-// e.g. pressure_pass.glslinc:80, coupled_motion.glslinc:402-408): the group
-// stages the partials in LDS in large chunks; lane 0 adds component CA and
-// lane 1 component CB (CB < 0: none) in index order, so the two dependent
-// chains advance in the same instructions, and the LDS reads of a batch are
-// issued ahead of its additions.  Lane 0 returns (sum CA, sum CB).
+// e.g. pressure_pass.glslinc:80, coupled_motion.glslinc:402-408).  The
+// partials are staged in LDS in chunks, each component in a contiguous run;
+// lane 0 adds component CA and lane 1 component CB (CB < 0: none) in index
+// order, so the two dependent chains advance in the same instructions.  The
+// lanes read their runs four values at a time and load the next batch of 32
+// before adding the current one, and while the first wave sums a chunk the
+// second wave stages the next (double buffer).  Lane 0 returns
+// (sum CA, sum CB).  Launched with WV_RED (two waves) threads.
+template <int C>
+__device__ __forceinline__ float component(float4 p) { return C == 0 ? p.x : (C == 1 ? p.y : (C == 2 ? p.z : p.w)); }
 template <int CA, int CB>
 __device__ __forceinline__ float2 serial_sums(const float4* partial, int64_t groups) {
-  constexpr int CHUNK = 2048, BATCH = 16;
-  __shared__ float4 stage[CHUNK];
-  const int lane = threadIdx.x;
-  const int comp = lane == 0 ? CA : CB;
-  float acc = 0;
-  for (int64_t base = 0; base < groups; base += CHUNK) {
-    __syncthreads();
-    for (int k = lane; k < CHUNK && base + k < groups; k += blockDim.x) stage[k] = partial[base + k];
-    __syncthreads();
-    if (lane < (CB >= 0 ? 2 : 1)) {
-      const float* src = reinterpret_cast<const float*>(stage) + comp;
-      int n = (int)(groups - base < CHUNK ? groups - base : CHUNK), k = 0;
-      for (; k + BATCH <= n; k += BATCH) {
-        float v[BATCH];
-#pragma unroll
-        for (int b = 0; b < BATCH; b++) v[b] = src[4 * (k + b)];
-#pragma unroll
-        for (int b = 0; b < BATCH; b++) acc += v[b];
-      }
-      for (; k < n; k++) acc += src[4 * k];
+  static_assert(WV_RED == 128, "one summing wave and one staging wave");
+  constexpr int CHUNK = 2048, NC = CB >= 0 ? 2 : 1, BATCH = 7, Q = CHUNK / 4;
+  __shared__ __align__(16) float stage[2][NC][CHUNK];
+  const int t = threadIdx.x;
+  const int chunks = (int)((groups + CHUNK - 1) / CHUNK);
+  auto fill = [&](int c, int first, int stride) {
+    const int64_t base = (int64_t)c * CHUNK;
+    const int n = (int)(groups - base < CHUNK ? groups - base : CHUNK);
+    for (int k = first; k < n; k += stride) {
+      const float4 p = partial[base + k];
+      stage[c & 1][0][k] = component<CA>(p);
+      if (NC == 2) stage[c & 1][NC - 1][k] = component<CB < 0 ? 0 : CB>(p);
     }
+  };
+  float acc = 0;
+  if (chunks > 0) fill(0, t, WV_RED);
+  __syncthreads();
+  for (int c = 0; c < chunks; c++) {
+    if (t >= 64) {
+      if (c + 1 < chunks) fill(c + 1, t - 64, WV_RED - 64);
+    } else if (t < NC) {
+      const float4* src = reinterpret_cast<const float4*>(stage[c & 1][t]);
+      const int64_t left = groups - (int64_t)c * CHUNK;
+      const int n = (int)(left < CHUNK ? left : CHUNK), n4 = n >> 2, nb = n4 / BATCH;
+      // batches past the last full one are loaded (inside the buffer) but not added
+      float4 a[BATCH], b[BATCH];
+#pragma unroll
+      for (int e = 0; e < BATCH; e++) a[e] = src[e];
+      for (int q = 0; q < nb; q += 2) {
+        const int jb = (q + 1) * BATCH < Q - BATCH ? (q + 1) * BATCH : Q - BATCH;
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) b[e] = src[jb + e];
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the additions
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) { acc += a[e].x; acc += a[e].y; acc += a[e].z; acc += a[e].w; }
+        if (q + 1 >= nb) break;
+        const int ja = (q + 2) * BATCH < Q - BATCH ? (q + 2) * BATCH : Q - BATCH;
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) a[e] = src[ja + e];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) { acc += b[e].x; acc += b[e].y; acc += b[e].z; acc += b[e].w; }
+      }
+      for (int j = nb * BATCH; j < n4; j++) { const float4 v = src[j]; acc += v.x; acc += v.y; acc += v.z; acc += v.w; }
+      for (int k = 4 * n4; k < n; k++) acc += stage[c & 1][t][k];
+    }
+    __syncthreads();
   }
   float other = __shfl(acc, 1, 64);
-  __syncthreads();
   return make_float2(acc, other);
 }
 
@@ -315,9 +345,12 @@ __global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, int final) {
   if (threadIdx.x == 0) { s.c[39] += g.x; s.c[65] += g.y; s.c[66] += g.z; }
 }
 // ------------------------------------------------ VOF update (25 + 26) ------
-__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int final) {
+// FINAL: the one-group finaliser, a kernel of its own so that the partial
+// pass does not carry the finaliser's LDS staging (occupancy)
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(1e20f, -1e20f, 0, 0);
   int sweep = (int)s.c[21];
   if (i < s.N) {
@@ -693,6 +726,12 @@ __global__ void __launch_bounds__(256) k_ext_hop(Ctx s, int rb) {
   I3 q; int axis; unpack_face(s, i, q, axis);
   float source = s.motion[source_base(s, rb) + i], best = 1e20f;
   if (!wall_face(s, q, axis)) {
+    // A face that is its own source sits at distance 0; a neighbour's source
+    // can only tie it from the same position, which another face of this
+    // axis has only across a periodic seam (z always, x when periodic), so
+    // off the seams the face keeps itself and the loop is skipped.
+    const bool seam = axis == 2 ? (q.z == 0 || q.z == s.nz) : (axis == 0 && s.perx > .5f && (q.x == 0 || q.x == s.nx));
+    if (source >= 0 && (int64_t)source == i && !seam) { s.motion[source_base(s, wb) + i] = source; return; }
     if (source >= 0) { I3 sq; int sa; unpack_face(s, (int64_t)source, sq, sa); F3 d = periodic_delta(s, q, sq); best = dot3(d, d); }
     for (int ax = 0; ax < 3; ax++) for (int sign = -1; sign <= 1; sign += 2) {
       I3 nb = q; addc(nb, ax, sign); nb = wrap_face(s, nb);
@@ -767,9 +806,10 @@ __global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, int final) {
   }
 }
 // -------------------------------------- diagnostics + acceptance (12 + 13) --
-__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int) {
   if (!post_ok(s)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 value = make_float4(0, 0, 0, 0); float dt = timestep(s);
   if (i < s.N) {
     int cb = candidate(s); I3 q = coord(i, dims(s)); float speed = 0;
@@ -831,6 +871,8 @@ static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) 
 // (a separate launch: the kernel boundary publishes the partials to every XCD)
 #define LRED(k, id) do { hipLaunchKernelGGL(k, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0); \
                          hipLaunchKernelGGL(k, dim3(1), dim3(WV_RED), 0, st, s, 1); } while (0)
+#define LRED2(k) do { hipLaunchKernelGGL(k<false>, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0); \
+                      hipLaunchKernelGGL(k<true>, dim3(1), dim3(WV_RED), 0, st, s, 1); } while (0)
 #define L1(k) hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, st, s)
 
 int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
@@ -847,7 +889,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 6: LRED(k_dot_alpha, 1); break;
   case 10: L256(k_direction, s.N); break;
   case 11: L256(k_project, s.NF); break;
-  case 12: LRED(k_accept, 2); break;
+  case 12: LRED2(k_accept); break;
   case 14: L256(k_export_plain, s.N); break;
   case 16: L256(k_coefficients, s.N); break;
   case 17: L256(k_reconstruct<0>, s.N); break;
@@ -861,7 +903,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 22: L256(k_bank_copy, s.stride); break;
   case 23: L256(k_reconstruct<1>, s.N); break;
   case 24: L256(k_flux, s.NF); break;
-  case 25: LRED(k_vof_update, 4); break;
+  case 25: LRED2(k_vof_update); break;
   case 27: case 28: {
     const bool corr = op == 28;
     if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }
