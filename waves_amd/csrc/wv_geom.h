@@ -87,25 +87,32 @@ __device__ __forceinline__ float coastal_rate(const Ctx& s, float x) {
   return s.forc[2] * gmax(left, s.right_abs * right);
 }
 
-// pressure_geometry.glslinc:39-47 (fractions of the geometry bank)
-__device__ __forceinline__ float geometry_fraction(const Ctx& s, I3 q, int gbank) {
+// pressure_geometry.glslinc:39-47 (fractions of the geometry bank).  The
+// *_known forms take solid(s, q), which a caller that has it already passes
+// in (solid() wraps q itself, so the flag of q and of its wrapped cell agree).
+__device__ __forceinline__ float geometry_fraction_known(const Ctx& s, I3 q, int gbank, bool sq) {
   q = wrap_cell(s, q);
-  if (q.x < 0 && !solid(s, q)) return coastal_fraction(s, q, s.c[1] + (gbank == candidate(s) ? timestep(s) : 0));
-  if (solid(s, q)) { q.x = iclamp(q.x, 0, s.nx - 1); q.y = iclamp(q.y, 0, s.ny - 1); q.z = iclamp(q.z, 0, s.nz - 1); }
+  if (q.x < 0 && !sq) return coastal_fraction(s, q, s.c[1] + (gbank == candidate(s) ? timestep(s) : 0));
+  if (sq) { q.x = iclamp(q.x, 0, s.nx - 1); q.y = iclamp(q.y, 0, s.ny - 1); q.z = iclamp(q.z, 0, s.nz - 1); }
   if (outside(s, q)) return 0.0f;
   return getv(s, gbank, index3(q, dims(s)));
 }
+__device__ __forceinline__ float geometry_fraction(const Ctx& s, I3 q, int gbank) { return geometry_fraction_known(s, q, gbank, solid(s, q)); }
 // coupled_motion.glslinc:36-44 (fractions of the current transport sweep)
-__device__ __forceinline__ float transport_f(const Ctx& s, I3 q) {
+__device__ __forceinline__ float transport_f_known(const Ctx& s, I3 q, bool sq) {
   q = wrap_cell(s, q);
-  if (q.x < 0 && !solid(s, q)) return coastal_fraction(s, q, s.c[1]);
-  if (outside(s, q) || solid(s, q)) return 0;
+  if (q.x < 0 && !sq) return coastal_fraction(s, q, s.c[1]);
+  if (outside(s, q) || sq) return 0;
   int64_t i = index3(q, dims(s)); int sweep = (int)s.c[21];
   return sweep == 0 ? getv(s, accepted(s), i) : (sweep == 1 ? getv(s, candidate(s), i) : s.motion[i]);
 }
+__device__ __forceinline__ float transport_f(const Ctx& s, I3 q) { return transport_f_known(s, q, solid(s, q)); }
 // source 0: geometry bank; source 1: transport sweep
 __device__ __forceinline__ float hf_fraction(const Ctx& s, I3 q, int source, int gbank) {
   return source == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+}
+__device__ __forceinline__ float hf_fraction_known(const Ctx& s, I3 q, int source, int gbank, bool sq) {
+  return source == 1 ? transport_f_known(s, q, sq) : geometry_fraction_known(s, q, gbank, sq);
 }
 
 // pressure_geometry.glslinc:72-90: weighted Youngs gradient, one-sided next to solids
@@ -118,7 +125,7 @@ __device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source,
 #pragma unroll
   for (int c = 0; c < 27; c++) {
     I3 p = mk(q.x + c % 3 - 1, q.y + (c / 3) % 3 - 1, q.z + c / 9 - 1);
-    sb[c] = solid(s, p); fb[c] = hf_fraction(s, p, source, gbank);
+    sb[c] = solid(s, p); fb[c] = hf_fraction_known(s, p, source, gbank, sb[c]);
   }
   F3 g = mkf(0, 0, 0);
 #pragma unroll
