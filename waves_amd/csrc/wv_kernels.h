@@ -36,6 +36,7 @@ struct MGDev {
   float* snapshot;           // 9 N: preconditioner's fine coefficients, kind, face signature
   float* previous;           // 9 N: previous trial's operator audit copy
   unsigned* state;           // hierarchy audit header + per-trial records
+  unsigned* audit_part;      // audit counts per block stripe (AUDIT_STRIPES x 16), summed into state[1..7, 15, 16]
   int lag, mode, capacity;
 };
 

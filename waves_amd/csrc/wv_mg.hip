@@ -322,46 +322,105 @@ __device__ __forceinline__ unsigned signature(const Ctx& s, int64_t i) {
   }
   return result;
 }
+constexpr int AUDIT_STRIPES = 64;  // stripes of 16 counters (a cache line each) in MGDev::audit_part
 __device__ __forceinline__ bool audit_on(const Ctx& s) { return !(s.c[5] < .5f || s.c[7] > .5f); }
 __global__ void __launch_bounds__(256) k_audit0(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
+  for (int k = threadIdx.x; k < 16 * AUDIT_STRIPES; k += blockDim.x) m.audit_part[k] = 0;
+  if (threadIdx.x != 0) return;
   unsigned* st = m.state;
   for (int k = 1; k <= 7; k++) st[k] = 0; st[14] = st[0]; st[15] = 0; st[16] = 0; for (int k = 20; k < 32; k++) st[k] = 0;
 }
+// Counts of the audit.  Each thread reads everything its cell needs up front
+// (the neighbours' kinds at a clamped index when outside, masked afterwards),
+// the wave and the block sum the counts (integers; the ratio as its float bits
+// under an unsigned maximum, as the atomic maximum of the reference takes it)
+// and one thread adds the block's totals into one of AUDIT_STRIPES padded
+// stripes, which k_audit2 sums into the header: the totals are those of the
+// per-cell atomics, without their contention on one cache line.
 __global__ void __launch_bounds__(256) k_audit1(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
-  __shared__ unsigned cnt[9];
-  if (threadIdx.x < 9) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  int64_t i = GID(), n = s.N;
+  __shared__ unsigned part[4][9];
+  // a_x, a_y, a_z, a_w, b_x, b_y, ratio bits (maximum), d_x, d_y
+  unsigned acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int64_t n = s.N, i = GID();
   if (i < n) {
     const float* cached = m.snapshot; const float* prev = m.previous;
-    float kind = s.geo[i].y;
-    unsigned a_x = kind != cached[7 * n + i], a_w = kind != prev[7 * n + i], d_y = kind > .5f;
-    unsigned sig = signature(s, i), old = (unsigned)cached[8 * n + i], pv = (unsigned)prev[8 * n + i];
-    unsigned a_y = 0, b_x = 0, a_z = 0, b_y = 0, d_x = 0; float ratio = 0;
-    for (int face = 0; face < 6; face++) { a_y += ((sig >> (2 * face)) & 3) != ((old >> (2 * face)) & 3); b_x += ((sig >> (2 * face)) & 3) != ((pv >> (2 * face)) & 3); }
-    for (int k = 0; k < 7; k++) {
-      float value = s.work[(6 + k) * n + i], was = cached[k * n + i];
-      a_z += __float_as_uint(value) != __float_as_uint(was); b_y += __float_as_uint(value) != __float_as_uint(prev[k * n + i]);
-      if (was > 0) ratio = gmax(ratio, fabsf(value - was) / was);
-      if (isnan(value) || isinf(value) || value < 0) d_x = 1;
+    const I3 dn = dims(s), q = coord(i, dn);
+    float value[7], was[7], pw[7], ngy[6];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { value[k] = s.work[(6 + k) * n + i]; was[k] = cached[k * n + i]; pw[k] = prev[k * n + i]; }
+    const float kind = s.geo[i].y, ck = cached[7 * n + i], pk = prev[7 * n + i];
+    const unsigned old = (unsigned)cached[8 * n + i], pv = (unsigned)prev[8 * n + i];
+    bool inside[6];
+#pragma unroll
+    for (int f6 = 0; f6 < 6; f6++) {
+      // pressure_hierarchy_state.glsl:60-68 (the signature's neighbour of face f6)
+      const int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
+      nb.z = pmod(nb.z, dn.z); if (s.perx > .5f) nb.x = pmod(nb.x, dn.x);
+      inside[f6] = nb.x >= 0 && nb.x < dn.x && nb.y >= 0 && nb.y < dn.y;
+      ngy[f6] = s.geo[inside[f6] ? index3(nb, dn) : i].y;
     }
-    if (a_x) atomicAdd(&cnt[0], a_x); if (a_y) atomicAdd(&cnt[1], a_y); if (a_z) atomicAdd(&cnt[2], a_z); if (a_w) atomicAdd(&cnt[3], a_w);
-    if (b_x) atomicAdd(&cnt[4], b_x); if (b_y) atomicAdd(&cnt[5], b_y); atomicMax(&cnt[6], __float_as_uint(ratio));
-    if (d_x) atomicAdd(&cnt[7], d_x); if (d_y) atomicAdd(&cnt[8], d_y);
+    unsigned sig = 0;
+    if (!(kind < .5f)) {
+#pragma unroll
+      for (int f6 = 0; f6 < 6; f6++) {
+        const bool w = inside[f6] && ngy[f6] > .5f;
+        const unsigned kd = value[f6] > 0 ? (w ? 1u : 2u) : 0u;
+        sig |= kd << (2 * f6);
+      }
+    }
+    acc[0] = kind != ck; acc[3] = kind != pk; acc[8] = kind > .5f;
+#pragma unroll
+    for (int face = 0; face < 6; face++) { acc[1] += ((sig >> (2 * face)) & 3) != ((old >> (2 * face)) & 3); acc[4] += ((sig >> (2 * face)) & 3) != ((pv >> (2 * face)) & 3); }
+    float ratio = 0; unsigned d_x = 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      acc[2] += __float_as_uint(value[k]) != __float_as_uint(was[k]); acc[5] += __float_as_uint(value[k]) != __float_as_uint(pw[k]);
+      if (was[k] > 0) ratio = gmax(ratio, fabsf(value[k] - was[k]) / was[k]);
+      if (isnan(value[k]) || isinf(value[k]) || value[k] < 0) d_x = 1;
+    }
+    acc[6] = __float_as_uint(ratio); acc[7] = d_x;
   }
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    unsigned v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      unsigned o = (unsigned)__shfl_xor((int)v, off, 64);
+      v = k == 6 ? (v > o ? v : o) : v + o;
+    }
+    acc[k] = v;
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 9; k++) part[wave][k] = acc[k];
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned* st = m.state;
-    for (int k = 0; k < 4; k++) if (cnt[k]) atomicAdd(&st[1 + k], cnt[k]);
-    if (cnt[4]) atomicAdd(&st[5], cnt[4]); if (cnt[5]) atomicAdd(&st[6], cnt[5]); atomicMax(&st[7], cnt[6]);
-    if (cnt[7]) atomicAdd(&st[15], cnt[7]); if (cnt[8]) atomicAdd(&st[16], cnt[8]);
+    unsigned* stripe = m.audit_part + 16 * (blockIdx.x % AUDIT_STRIPES);
+    for (int k = 0; k < 9; k++) {
+      unsigned v = part[0][k];
+      for (int w = 1; w < 4; w++) v = k == 6 ? (v > part[w][k] ? v : part[w][k]) : v + part[w][k];
+      if (k == 6) atomicMax(&stripe[k], v); else if (v) atomicAdd(&stripe[k], v);
+    }
   }
 }
 __global__ void __launch_bounds__(256) k_audit2(Ctx s, MGDev m) {
   if (!audit_on(s)) return;
   unsigned* st = m.state;
+  {  // the stripes of k_audit1 into the header (one stripe per lane)
+    const int lane = threadIdx.x;
+    unsigned v[9];
+    for (int k = 0; k < 9; k++) v[k] = lane < AUDIT_STRIPES ? m.audit_part[16 * lane + k] : 0u;
+    for (int k = 0; k < 9; k++)
+      for (int off = 32; off > 0; off >>= 1) {
+        unsigned o = (unsigned)__shfl_xor((int)v[k], off, 64);
+        v[k] = k == 6 ? (v[k] > o ? v[k] : o) : v[k] + o;
+      }
+    if (lane != 0) return;
+    for (int k = 0; k < 4; k++) st[1 + k] += v[k];
+    st[5] += v[4]; st[6] += v[5]; st[7] = st[7] > v[6] ? st[7] : v[6]; st[15] += v[7]; st[16] += v[8];
+  }
   if (st[15] > 0) { s.c[7] = 34; return; }
   int mode = m.mode;
   bool incompatible = st[0] == 0 || st[1] > 0 || st[2] > 0;
@@ -679,11 +738,24 @@ __global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
   }
 }
 // pressure_mg_coarse.glslinc:35-49 (op 11): dense Cholesky in LDS, store L^-1
-__global__ void __launch_bounds__(256) k_mg_cholesky(Ctx s, MGDev m, int l) {
+// Dense Cholesky factor of the coarsest level and the inverse of the factor
+// (pressure_multigrid.glslinc, coarsest-level solve).  Written right-looking:
+// once column k is final every entry below and to the right subtracts its
+// product with column k, so each entry receives exactly the subtractions of
+// the left-looking loops (a - L_i0 L_k0 - L_i1 L_k1 - ..., same operands,
+// same j order) while the updates of one step run on all threads.  Every
+// thread takes the square root of the diagonal and the quotients of column k
+// it needs itself (the same operations on the same values, so the same bits
+// as the stored factor), which leaves one barrier per step; row k - 1 of the
+// inverse of the factor, built the same way, shares step k.
+constexpr int CHOL_THREADS = 256;
+__global__ void __launch_bounds__(CHOL_THREADS) k_mg_cholesky(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
-  __shared__ float dense[64 * 64], inv[64 * 64];
-  int lid = threadIdx.x, N = m.lv[l].count; const signed char* slots = m.slots + 125 * l;
-  for (int k = lid; k < N * N; k += blockDim.x) {
+  // dense: the working matrix (lower triangle); lf: the factor; acc: rows of
+  // the inverse still accumulating
+  __shared__ float dense[64 * 64], lf[64 * 64], acc[64 * 64];
+  const int lid = threadIdx.x, N = m.lv[l].count; const signed char* slots = m.slots + 125 * l;
+  for (int k = lid; k < N * N; k += CHOL_THREADS) {
     int i = k / N, j = k % N; float v;
     if (!mactive(m, l, i) || !mactive(m, l, j)) v = i == j ? 1 : 0;
     else {
@@ -692,32 +764,43 @@ __global__ void __launch_bounds__(256) k_mg_cholesky(Ctx s, MGDev m, int l) {
       v = sl < 0 ? 0 : m.rows[m.lv[l].row_off + (int64_t)sl * N + a];
     }
     dense[k] = v;
+    acc[k] = i == j ? 1 : 0;
   }
   __syncthreads();
-  for (int k = 0; k < N; k++) {
-    if (lid == 0) {
-      float d = dense[k * N + k]; for (int j = 0; j < k; j++) d -= dense[k * N + j] * dense[k * N + j];
-      if (!(d > 0) || isnan(d) || isinf(d)) { s.c[7] = 33; d = 1; }
-      dense[k * N + k] = sqrtf(d);
+  const int wave = lid >> 6, lane = lid & 63;
+  for (int k = 0; k <= N; k++) {
+    if (k < N) {
+      float d = dense[k * N + k];
+      const bool bad = !(d > 0) || isnan(d) || isinf(d);
+      if (bad) d = 1;
+      const float lkk = sqrtf(d);
+      if (lid == 0) { if (bad) s.c[7] = 33; lf[k * N + k] = lkk; }
+      if (lid > k && lid < N) lf[lid * N + k] = dense[lid * N + k] / lkk;
+      const int c = k + 1 + lane;
+      if (c < N) {
+        const float lc = dense[c * N + k] / lkk;
+        for (int r = c + ((wave - (c - k - 1)) & 3); r < N; r += CHOL_THREADS / 64)
+          dense[r * N + c] -= dense[r * N + k] / lkk * lc;
+      }
+    }
+    if (k >= 1 && lane < N) {
+      const int row = k - 1;
+      const float x = acc[row * N + lane] / lf[row * N + row];
+      if (wave == 0) m.linv[row * N + lane] = x;
+      for (int r = row + 1 + wave; r < N; r += CHOL_THREADS / 64)
+        acc[r * N + lane] -= lf[r * N + row] * x;
     }
     __syncthreads();
-    if (lid > k && lid < N) { float d = dense[lid * N + k]; for (int j = 0; j < k; j++) d -= dense[lid * N + j] * dense[k * N + j]; dense[lid * N + k] = d / dense[k * N + k]; }
-    __syncthreads();
   }
-  for (int row = 0; row < N; row++) {
-    if (lid < N) { float d = lid == row ? 1 : 0; for (int j = 0; j < row; j++) d -= dense[row * N + j] * inv[j * N + lid]; inv[row * N + lid] = d / dense[row * N + row]; }
-    __syncthreads();
-  }
-  for (int k = lid; k < N * N; k += blockDim.x) m.linv[k] = inv[k];
 }
 
 static inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // water/pressure_multigrid.gd:121-128 + water/pressure_hierarchy_state.gd:36-41
 void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
-  hipLaunchKernelGGL(k_audit0, dim3(1), dim3(1), 0, st, s, m);
+  hipLaunchKernelGGL(k_audit0, dim3(1), dim3(256), 0, st, s, m);
   hipLaunchKernelGGL(k_audit1, dim3(nblocks(s.N, 256)), dim3(256), 0, st, s, m);
-  hipLaunchKernelGGL(k_audit2, dim3(1), dim3(1), 0, st, s, m);
+  hipLaunchKernelGGL(k_audit2, dim3(1), dim3(64), 0, st, s, m);
   hipLaunchKernelGGL(k_audit3, dim3(nblocks(s.N, 256)), dim3(256), 0, st, s, m);
   for (int l = 0; l < m.L; l++) hipLaunchKernelGGL(k_mg_classify, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l);
   hipLaunchKernelGGL(k_mg_l1_fine, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m);
@@ -735,7 +818,7 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks((int64_t)m.lv[l + 1].count * SYM_LANES, 256)), dim3(256), 0, st, s, m, l + 1);
     hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
   }
-  if (m.lv[m.L - 1].count <= 64) hipLaunchKernelGGL(k_mg_cholesky, dim3(1), dim3(64), 0, st, s, m, m.L - 1);
+  if (m.lv[m.L - 1].count <= 64) hipLaunchKernelGGL(k_mg_cholesky, dim3(1), dim3(CHOL_THREADS), 0, st, s, m, m.L - 1);
   hipLaunchKernelGGL(k_audit4, dim3(1), dim3(1), 0, st, s, m);
 }
 

@@ -293,6 +293,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(m.snapshot, 9 * c.N);
   ALLOC(m.previous, 9 * c.N);
   ALLOC(m.state, 64 + 32 * s->history_capacity);
+  ALLOC(m.audit_part, 64 * 16);
 #undef ALLOC
   m.offsets = offs; m.slots = slots; m.deltas = deltas;
   m.lag = (int)o[2]; m.mode = (int)o[4]; m.capacity = (int)s->history_capacity;
