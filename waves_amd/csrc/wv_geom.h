@@ -205,46 +205,61 @@ __device__ __forceinline__ float segment_distance(F3 p, F3 a, F3 b) {
   float t = gclamp(dot3(pa, ab) / gmax(dot3(ab, ab), 1e-30f), 0, 1);
   return len3(mkf(p.x - (a.x + t * ab.x), p.y - (a.y + t * ab.y), p.z - (a.z + t * ab.z)));
 }
+// Vertex lists of the polygon routines (at most 12 vertices), one list per
+// thread in LDS: coordinate c of vertex j at p[(3 j + c) * STRIDE] with p the
+// thread's slot, so consecutive threads use consecutive banks.  As private
+// arrays indexed by a running count they would live in scratch memory.
+template <int STRIDE>
+struct Verts {
+  float* p;
+  __device__ __forceinline__ F3 get(int j) const { return mkf(p[(3 * j) * STRIDE], p[(3 * j + 1) * STRIDE], p[(3 * j + 2) * STRIDE]); }
+  __device__ __forceinline__ void put(int j, F3 a) const { p[(3 * j) * STRIDE] = a.x; p[(3 * j + 1) * STRIDE] = a.y; p[(3 * j + 2) * STRIDE] = a.z; }
+};
+constexpr int VERT_FLOATS = 36;  // floats of one list per thread
 // pressure_geometry.glslinc:105-120: distance to the PLIC polygon of a cell
-__device__ __forceinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq) {
+template <int STRIDE>
+__device__ __forceinline__ float polygon_distance(const Ctx& s, F3 point, float4 eq, const Verts<STRIDE>& vert) {
   F3 h = mkf(s.h[0], s.h[1], s.h[2]);
   F3 normal = mkf(eq.x / h.x, eq.y / h.y, eq.z / h.z);
   float norm2 = dot3(normal, normal), step = dot3(normal, point) - eq.w;
   float signed_d = (dot3(normal, point) - eq.w) / sqrtf(norm2);
   F3 pr = mkf(point.x - normal.x * step / norm2, point.y - normal.y * step / norm2, point.z - normal.z * step / norm2);
   if (pr.x >= 0 && pr.y >= 0 && pr.z >= 0 && pr.x <= h.x && pr.y <= h.y && pr.z <= h.z) return fabsf(signed_d);
-  F3 vert[12]; int count = 0;
+  int count = 0;
   float e3[3] = {eq.x, eq.y, eq.z};
   for (int axis = 0; axis < 3; axis++) for (int j = 0; j < 2; j++) for (int k = 0; k < 2; k++) {
     F3 a = mkf(0, 0, 0); setc(a, (axis + 1) % 3, (float)j); setc(a, (axis + 2) % 3, (float)k);
     float den = e3[axis]; if (fabsf(den) < 1e-15f) continue;
     float t = (eq.w - dot3(mkf(eq.x, eq.y, eq.z), a)) / den;
-    if (t >= 0 && t <= 1) { setc(a, axis, t); vert[count++] = mkf(a.x * h.x, a.y * h.y, a.z * h.z); }
+    if (t >= 0 && t <= 1) { setc(a, axis, t); vert.put(count++, mkf(a.x * h.x, a.y * h.y, a.z * h.z)); }
   }
   float best = 1e10f;
   for (int j = 0; j < count; j++) {
-    best = gmin(best, len3(mkf(point.x - vert[j].x, point.y - vert[j].y, point.z - vert[j].z)));
-    for (int k = 0; k < j; k++) best = gmin(best, segment_distance(point, vert[j], vert[k]));
+    const F3 vj = vert.get(j);
+    best = gmin(best, len3(mkf(point.x - vj.x, point.y - vj.y, point.z - vj.z)));
+    for (int k = 0; k < j; k++) best = gmin(best, segment_distance(point, vj, vert.get(k)));
   }
   return best;
 }
-// pressure_geometry.glslinc:124-132
-__device__ __forceinline__ int clip_polygon(F3* poly, int count, float4 eq) {
-  F3 result[12]; int size = 0; F3 n = mkf(eq.x, eq.y, eq.z);
+// pressure_geometry.glslinc:124-132: the polygon in `poly` clipped by a plane
+// into `out` (the reference clips in place through a copy: same vertices)
+template <int STRIDE>
+__device__ __forceinline__ int clip_polygon(const Verts<STRIDE>& poly, int count, float4 eq, const Verts<STRIDE>& out) {
+  int size = 0; F3 n = mkf(eq.x, eq.y, eq.z);
   for (int j = 0; j < count; j++) {
-    F3 a = poly[j], b = poly[(j + 1) % count];
+    F3 a = poly.get(j), b = poly.get((j + 1) % count);
     float da = dot3(n, a) - eq.w, db = dot3(n, b) - eq.w;
-    if (da <= 0) result[size++] = a;
-    if ((da < 0 && db > 0) || (da > 0 && db < 0)) {
+    if (da <= 0 && size < 12) out.put(size++, a);
+    if (((da < 0 && db > 0) || (da > 0 && db < 0)) && size < 12) {
       float t = da / (da - db);
-      result[size++] = mkf(a.x * (1 - t) + b.x * t, a.y * (1 - t) + b.y * t, a.z * (1 - t) + b.z * t);
+      out.put(size++, mkf(a.x * (1 - t) + b.x * t, a.y * (1 - t) + b.y * t, a.z * (1 - t) + b.z * t));
     }
   }
-  for (int j = 0; j < size; j++) poly[j] = result[j];
   return size;
 }
 // pressure_geometry.glslinc:133-156: represented phase transitions on a cell face
-__device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 left, int axis, int gbank) {
+template <int STRIDE>
+__device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 left, int axis, int gbank, const Verts<STRIDE>& poly, const Verts<STRIDE>& spare) {
   I3 right = left; addc(right, axis, 1);
   I3 l = wrap_cell(s, left), r = wrap_cell(s, right);
   if (outside(s, l) || outside(s, r) || solid(s, l) || solid(s, r)) return 1e10f;
@@ -256,11 +271,11 @@ __device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 
   b.w += axis == 0 ? b.x : (axis == 1 ? b.y : b.z);
   float best = 1e10f; F3 h = mkf(s.h[0], s.h[1], s.h[2]);
   for (int side = 0; side < 2; side++) {
-    F3 poly[12]; int j = (axis + 1) % 3, k = (axis + 2) % 3;
-    for (int v = 0; v < 4; v++) { poly[v] = mkf(0, 0, 0); setc(poly[v], axis, 1); setc(poly[v], j, (v == 1 || v == 2) ? 1 : 0); setc(poly[v], k, v >= 2 ? 1 : 0); }
+    int j = (axis + 1) % 3, k = (axis + 2) % 3;
+    for (int v = 0; v < 4; v++) { F3 c = mkf(0, 0, 0); setc(c, axis, 1); setc(c, j, (v == 1 || v == 2) ? 1 : 0); setc(c, k, v >= 2 ? 1 : 0); poly.put(v, c); }
     float4 first = side == 0 ? a : b;
     float4 second = side == 0 ? make_float4(-b.x, -b.y, -b.z, -b.w) : make_float4(-a.x, -a.y, -a.z, -a.w);
-    int count = clip_polygon(poly, 4, first); count = clip_polygon(poly, count, second);
+    int count = clip_polygon(poly, 4, first, spare); count = clip_polygon(spare, count, second, poly);
     if (count == 0) continue;
     F3 projected = point; setc(projected, axis, compc(h, axis));
     F3 local = mkf(projected.x / h.x, projected.y / h.y, projected.z / h.z);
@@ -268,7 +283,7 @@ __device__ __forceinline__ float transition_distance(const Ctx& s, F3 point, I3 
         dot3(mkf(first.x, first.y, first.z), local) <= first.w && dot3(mkf(second.x, second.y, second.z), local) <= second.w)
       best = gmin(best, fabsf(compc(point, axis) - compc(h, axis)));
     for (int v = 0; v < count; v++) {
-      F3 p0 = poly[v], p1 = poly[(v + 1) % count];
+      F3 p0 = poly.get(v), p1 = poly.get((v + 1) % count);
       best = gmin(best, segment_distance(point, mkf(p0.x * h.x, p0.y * h.y, p0.z * h.z), mkf(p1.x * h.x, p1.y * h.y, p1.z * h.z)));
     }
   }
@@ -297,7 +312,8 @@ __device__ __forceinline__ float distance_head(const Ctx& s, I3 q, int gbank, bo
 // Neighbour k (x-major over the 3x3x3 block, the search order) of a seeded
 // cell: the distance to its box and the least of its polygon and transition
 // distances; +inf where the neighbour is outside or solid.
-__device__ __forceinline__ void distance_neighbour(const Ctx& s, I3 q, int k, int gbank, float& box, float& m) {
+template <int STRIDE>
+__device__ __forceinline__ void distance_neighbour(const Ctx& s, I3 q, int k, int gbank, float& box, float& m, const Verts<STRIDE>& va, const Verts<STRIDE>& vb) {
   const float inf = __int_as_float(0x7f800000);
   box = inf; m = inf;
   I3 nbr = mk(q.x + k / 9 - 1, q.y + (k / 3) % 3 - 1, q.z + k % 3 - 1), wr = wrap_cell(s, nbr);
@@ -308,8 +324,8 @@ __device__ __forceinline__ void distance_neighbour(const Ctx& s, I3 q, int k, in
   F3 d = mkf(point.x - gclamp(point.x, 0, h.x), point.y - gclamp(point.y, 0, h.y), point.z - gclamp(point.z, 0, h.z));
   box = len3(d);
   float c = inf;
-  if (fraction > 0 && fraction < 1) c = gmin(c, polygon_distance(s, point, s.plane[index3(wr, dims(s))]));
-  for (int axis = 0; axis < 3; axis++) c = gmin(c, transition_distance(s, point, nbr, axis, gbank));
+  if (fraction > 0 && fraction < 1) c = gmin(c, polygon_distance(s, point, s.plane[index3(wr, dims(s))], va));
+  for (int axis = 0; axis < 3; axis++) c = gmin(c, transition_distance(s, point, nbr, axis, gbank, va, vb));
   m = c;
 }
 // pressure_geometry.glslinc:157-182.  The search keeps a running best and

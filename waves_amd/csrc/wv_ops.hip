@@ -135,15 +135,18 @@ __global__ void __launch_bounds__(256) k_distance_direct(Ctx s) {
 }
 // lane k < 27 of a 32-lane group evaluates neighbour k; lane 0 gathers the 27
 // (box, candidate) pairs and runs the reference's pruned search in order
-__global__ void __launch_bounds__(256) k_distance_deferred(Ctx s) {
+constexpr int DIST_THREADS = 128;  // threads per block of the queued distance pass
+__global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s) {
   if (s.c[4] > .5f && s.c[5] < .5f) return;
+  __shared__ float verts[2 * VERT_FLOATS * DIST_THREADS];
+  const Verts<DIST_THREADS> va{verts + threadIdx.x}, vb{verts + VERT_FLOATS * DIST_THREADS + threadIdx.x};
   const int64_t n = s.mq_count[2], groups = (int64_t)gridDim.x * blockDim.x / 32;
   const int sub = (int)(threadIdx.x & 31);
   const int gb = geometry_bank(s, false);
   for (int64_t j = GID() / 32; j < n; j += groups) {
     int64_t i = s.mq[j]; I3 q = coord(i, dims(s));
     float box = __int_as_float(0x7f800000), m = box;
-    if (sub < 27) distance_neighbour(s, q, sub, gb, box, m);
+    if (sub < 27) distance_neighbour(s, q, sub, gb, box, m, va, vb);
     bool seed; float head = distance_head(s, q, gb, seed);
     float best = fabsf(head);  // 4 |h|, the search's start
     // every lane runs the same in-order search over the gathered pairs
@@ -896,7 +899,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 18:
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
     L256(k_distance_direct, s.N);
-    hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s);
     break;
   case 19: L256(k_reconstruct<2>, s.N); break;
   case 20: LRED(k_cfl, 3); break;
