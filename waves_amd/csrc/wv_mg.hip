@@ -867,6 +867,19 @@ static int tail_start(const MGDev& m) {
   }
   return best;
 }
+// dst[e] = src(e) for e < n over the workgroup, eight elements per thread in
+// flight (all loads of a round before its stores)
+template <class Src>
+__device__ __forceinline__ void tail_copy(float* dst, int n, Src src) {
+  constexpr int U = 8;
+  for (int base = threadIdx.x; base < n; base += U * TAIL_THREADS) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) { const int e = base + u * TAIL_THREADS; v[u] = src(e < n ? e : n - 1); }  // in-range loads, unconditional
+#pragma unroll
+    for (int u = 0; u < U; u++) { const int e = base + u * TAIL_THREADS; if (e < n) dst[e] = v[u]; }
+  }
+}
 template <bool H>
 __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLayout tl, int phase, int smoothing) {
   if (!mg_enabled(s, phase)) return;
@@ -876,8 +889,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
   auto active = [&](int l, int i) { return i >= 0 && vec(l, 3)[i] > .5f; };
   for (int l = tl.T; l < tl.L; l++) {
     const MGLevel& L = m.lv[l]; const int n = L.count, W = L.width;
-    for (int e = tid; e < W * n; e += TAIL_THREADS) sm[tl.coef[l] + e] = H ? bf16_to_f32(m.rows16[L.row_off + e]) : m.rows[L.row_off + e];
-    for (int e = tid; e < 8 * n; e += TAIL_THREADS) sm[tl.vec[l] + e] = m.vec[L.vec_off + e];
+    const int64_t ro = L.row_off, vo = L.vec_off;
+    if (H) tail_copy(sm + tl.coef[l], W * n, [&](int e) { return bf16_to_f32(m.rows16[ro + e]); });
+    else tail_copy(sm + tl.coef[l], W * n, [&](int e) { return m.rows[ro + e]; });
+    tail_copy(sm + tl.vec[l], 8 * n, [&](int e) { return m.vec[vo + e]; });
     for (int k = tid; k < W; k += TAIL_THREADS) {
       int* o = reinterpret_cast<int*>(sm) + tl.off[l] + 4 * k;
       o[0] = m.offsets[3 * (L.off_start + k)]; o[1] = m.offsets[3 * (L.off_start + k) + 1];
@@ -885,7 +900,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
     }
   }
   const int nl = m.lv[tl.L - 1].count;
-  for (int e = tid; e < nl * nl; e += TAIL_THREADS) sm[tl.linv + e] = m.linv[e];
+  tail_copy(sm + tl.linv, nl * nl, [&](int e) { return m.linv[e]; });
   __syncthreads();
   // A x_source at row i of level l, entries in stencil order (coarse_apply;
   // on wrap1 levels the neighbour is row + delta with one periodic correction)
@@ -1025,7 +1040,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
   }
   for (int l = tl.T; l < tl.L; l++) {
     const MGLevel& L = m.lv[l];
-    for (int e = tid; e < 8 * L.count; e += TAIL_THREADS) m.vec[L.vec_off + e] = sm[tl.vec[l] + e];
+    tail_copy(m.vec + L.vec_off, 8 * L.count, [&](int e) { return sm[tl.vec[l] + e]; });
   }
 }
 // the field holding level T's result after the tail (the host's replay of the
