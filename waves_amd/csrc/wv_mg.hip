@@ -192,9 +192,15 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
       if ((a == 2 || (a == 0 && m.perx)) && nfa[a] <= 4) { lo[a] = 0; ext[a] = nfa[a]; }
     }
     int total = ext[0] * ext[1] * ext[2];
+    // the flag and value loads of a child do not wait on its flag (in-range
+    // index for an outside child, which adds nothing), so the loads of
+    // several children are in flight together
+#pragma unroll 4
     for (int t = rc.lane; t < total; t += rc.G) {
       int x = t % ext[0], y = (t / ext[0]) % ext[1], z = t / (ext[0] * ext[1]);
-      int j = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)); if (!mactive(m, parent, j)) continue;
+      const int jw = mwrap(m, parent, mk(lo[0] + x, lo[1] + y, lo[2] + z)), j = jw >= 0 ? jw : 0;
+      const float flag = m.vec[at(m, parent, 3, j)], v = m.vec[at(m, parent, 6, j)];
+      const bool on = jw >= 0 && flag > .5f;  // mactive(m, parent, jw)
       float weight = 1;
       if (!P.aggregate) {
         I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z};
@@ -210,7 +216,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
           weight *= w;
         }
       }
-      sum += weight * m.vec[at(m, parent, 6, j)];
+      if (on) sum += weight * v;
     }
   }
   sum = group_sum(sum, rc.G);
