@@ -547,23 +547,39 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l, i
     } else {
       I3 o = moffset(m, l, k); off[0] = o.x; off[1] = o.y; off[2] = o.z;
       j = mwrap(m, l, mk(q.x + o.x, q.y + o.y, q.z + o.z));
-      aij = j < 0 ? 0 : m.rows[F.row_off + (int64_t)k * F.count + i];
+      const float a_row = m.rows[F.row_off + (int64_t)k * F.count + i];  // loaded before j is known
+      aij = j < 0 ? 0 : a_row;
     }
-    if (j < 0 || aij == 0 || !mactive(m, l, j)) continue;
+    // the neighbour's flag at an in-range index, loaded with the coefficient
+    // (mactive(m, l, j) for j >= 0)
+    const float fj = m.vec[at(m, l, 3, j >= 0 ? j : 0)];
+    if (j < 0 || aij == 0 || !(fj > .5f)) continue;
     int ju[3] = {qa[0] + off[0], qa[1] + off[1], qa[2] + off[2]};  // unwrapped neighbour
     int base[3]; float t[3];
     for (int a = 0; a < 3; a++) {
       if (F.aggregate || F.f[a] == 1) { base[a] = parent_base(F, a, ju[a]); t[a] = 0; }
       else { float pos = ((float)ju[a] + .5f) / (float)F.f[a] - .5f; base[a] = (int)floorf(pos); t[a] = fract(pos); }
     }
-    int nb0 = F.aggregate ? 1 : F.f[0], nb1 = F.aggregate ? 1 : F.f[1], nb2 = F.aggregate ? 1 : F.f[2];
-    for (int z = 0; z < nb2; z++) for (int y = 0; y < nb1; y++) for (int x = 0; x < nb0; x++) {
+    // at most 2 x 2 x 2 parents (coarsening factors are 1 or 2): their
+    // indices and weights first, then all their flags loaded together, then
+    // the accumulation in the same x-fastest order
+    const int nb0 = F.aggregate ? 1 : F.f[0], nb1 = F.aggregate ? 1 : F.f[1], nb2 = F.aggregate ? 1 : F.f[2];
+    int Jc[8], bc[8]; float wc[8], fc[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
       int bit[3] = {x, y, z}; float w = 1; int P[3];
+#pragma unroll
       for (int a = 0; a < 3; a++) { P[a] = base[a] + bit[a]; if (!F.aggregate && F.f[a] > 1) w *= bit[a] == 1 ? t[a] : 1 - t[a]; }
-      int J = parent_node(m, lc, P);
-      if (J < 0 || w == 0 || !mactive(m, lc, J)) continue;
-      int b = (P[0] - anchor[0]) + E0 * ((P[1] - anchor[1]) + E1 * (P[2] - anchor[2]));
-      acc[b * 128 + tid] += aij * w;
+      const bool in = x < nb0 && y < nb1 && z < nb2;
+      const int J = in ? parent_node(m, lc, P) : -1;
+      Jc[c] = J; wc[c] = w; bc[c] = (P[0] - anchor[0]) + E0 * ((P[1] - anchor[1]) + E1 * (P[2] - anchor[2]));
+      fc[c] = m.vec[at(m, lc, 3, J >= 0 ? J : 0)];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      if (Jc[c] < 0 || wc[c] == 0 || !(fc[c] > .5f)) continue;
+      acc[bc[c] * 128 + tid] += aij * wc[c];
     }
   }
   }
