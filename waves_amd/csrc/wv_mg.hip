@@ -604,7 +604,11 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
   int64_t I = gid >> (__ffs(G) - 1); int lane = (int)(gid & (G - 1));
   int W = C.width;
   for (int k = 0; k < W; k++) acc[k * GAL_THREADS + tid] = 0;
-  const signed char* slots = m.slots + 125 * lc;
+  // the level's slot table (delta -> stencil entry) in LDS: looked up once per
+  // box entry, right before the accumulator update that needs it
+  __shared__ signed char slots[128];
+  for (int k = tid; k < 125; k += GAL_THREADS) slots[k] = m.slots[125 * lc + k];
+  __syncthreads();
   bool live = I < C.count && mactive(m, lc, (int)I);
   int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2], E = E0 * E1 * E2;
   if (live) {
