@@ -746,11 +746,17 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int 
 // SYM_LANES lanes per row, each taking every SYM_LANES-th entry: the entries
 // are independent (a mirrored entry reads a canonical one, never written here)
 #define SYM_LANES 8
+// Consecutive threads take consecutive rows of the same entry group, so the
+// k-major rows are read and written coalesced; the slot table is in LDS.
 __global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
-  int64_t gid = GID(); const MGLevel& C = m.lv[l];
-  int64_t I = gid / SYM_LANES; int lane = (int)(gid % SYM_LANES); if (I >= C.count) return;
-  I3 q = coord(I, mdims(m, l)); const signed char* slots = m.slots + 125 * l;
+  __shared__ signed char slots[128];
+  for (int k = threadIdx.x; k < 125; k += blockDim.x) slots[k] = m.slots[125 * l + k];
+  __syncthreads();
+  const MGLevel& C = m.lv[l];
+  const uint32_t gid = (uint32_t)GID(), n = (uint32_t)C.count;
+  const int lane = (int)(gid / n); const int64_t I = gid - (uint32_t)lane * n; if (lane >= SYM_LANES) return;
+  I3 q = coord(I, mdims(m, l));
   float* base = m.rows + C.row_off;
   for (int k = lane; k < C.width; k += SYM_LANES) {
     I3 off = moffset(m, l, k); int J = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
