@@ -491,7 +491,17 @@ __global__ void __launch_bounds__(256) k_mg_l1_coarse(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
   int64_t i = GID(); int N = m.lv[l].count; if (i >= N) return;
   const float* row = m.rows + m.lv[l].row_off + i; float sum = 0;
-  for (int k = 0; k < m.lv[l].width; k++) sum += fabsf(row[(int64_t)k * N]);
+  // the row's entries in order, eight loads in flight ahead of their additions
+  const int W = m.lv[l].width;
+  int k = 0;
+  for (; k + 8 <= W; k += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = row[(int64_t)(k + u) * N];
+#pragma unroll
+    for (int u = 0; u < 8; u++) sum += fabsf(v[u]);
+  }
+  for (; k < W; k++) sum += fabsf(row[(int64_t)k * N]);
   if (!(sum > 0) || isnan(sum) || isinf(sum)) { s.c[7] = 32; return; }
   m.vec[at(m, l, 4, i)] = 1 / sum;
 }
