@@ -471,7 +471,10 @@ __global__ void __launch_bounds__(256) k_mg_classify(Ctx s, MGDev m, int l) {
     for (int z = 0; z < P.f[2]; z++) for (int y = 0; y < P.f[1]; y++) for (int x = 0; x < P.f[0]; x++)
       if (mactive(m, l - 1, mwrap(m, l - 1, mk(base.x + x, base.y + y, base.z + z)))) value = 1;
   }
-  if (m.vec[at(m, l, 3, i)] != value) atomicAdd(&m.state[20 + l], 1u);
+  // changed flags counted per wave, one atomic per wave (an integer count)
+  const bool changed = m.vec[at(m, l, 3, i)] != value;
+  const unsigned long long ballot = __ballot(changed);
+  if (changed && __lane_id() == __ffsll((long long)ballot) - 1) atomicAdd(&m.state[20 + l], (unsigned)__popcll(ballot));
   m.vec[at(m, l, 3, i)] = value;
 }
 // pressure_multigrid.glslinc:183-189 (op 4), level 0
@@ -479,10 +482,17 @@ __global__ void __launch_bounds__(256) k_mg_l1_fine(Ctx s, MGDev m) {
   if (!build_on(s, m, false)) return;
   int64_t i = GID(); if (i >= m.lv[0].count) return;
   I3 q = coord(i, mdims(m, 0)); float sum = snap(m, 12, i);
+  // the six neighbours' flags and coefficients loaded together (in-range
+  // index for an outside neighbour), then added in face order
+  float flag[6], cf[6]; bool in[6];
+#pragma unroll
   for (int f6 = 0; f6 < 6; f6++) {
     int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
-    if (mactive(m, 0, mwrap(m, 0, nb))) sum += snap(m, 6 + f6, i);
+    const int j = mwrap(m, 0, nb); in[f6] = j >= 0;
+    flag[f6] = m.vec[at(m, 0, 3, j >= 0 ? j : 0)]; cf[f6] = snap(m, 6 + f6, i);
   }
+#pragma unroll
+  for (int f6 = 0; f6 < 6; f6++) if (in[f6] && flag[f6] > .5f) sum += cf[f6];  // mactive(m, 0, j)
   if (!(sum > 0) || isnan(sum) || isinf(sum)) { s.c[7] = 32; return; }
   m.vec[at(m, 0, 4, i)] = 1 / sum;
 }
