@@ -242,7 +242,34 @@ __global__ void __launch_bounds__(256) k_mg_residual(Ctx s, MGDev m, int l, int 
 // op 9: x_l += P x_{l+1}
 __device__ __forceinline__ void prolong_row(const MGDev& m, int l, int ff, int cf, int64_t i) {
   float add = 0;
-  if (mactive(m, l, (int)i)) { int ids[8]; float ws[8]; int size = mtransfers(m, l, (int)i, ids, ws); for (int k = 0; k < size; k++) add += ws[k] * m.vec[at(m, l + 1, cf, ids[k])]; }
+  const MGLevel& L = m.lv[l];
+  if (L.aggregate) {
+    if (mactive(m, l, (int)i)) { int ids[8]; float ws[8]; int size = mtransfers(m, l, (int)i, ids, ws); for (int k = 0; k < size; k++) add += ws[k] * m.vec[at(m, l + 1, cf, ids[k])]; }
+  } else {
+    // mtransfers' parents (at most 2 x 2 x 2: factors are 1 or 2) with their
+    // flags and values loaded together, then added in its order (x fastest)
+    // under its conditions
+    const bool on = m.vec[at(m, l, 3, i)] > .5f;  // mactive(m, l, i)
+    const I3 n = mdims(m, l + 1), q = coord(i, mdims(m, l));
+    int base[3]; float t[3]; const int qq[3] = {q.x, q.y, q.z};
+    for (int k = 0; k < 3; k++) { float pos = ((float)qq[k] + .5f) / (float)L.f[k] - .5f; base[k] = (int)floorf(pos); t[k] = fract(pos); }
+    float wc[8], fc[8], vc[8]; bool use[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int bit[3] = {c & 1, (c >> 1) & 1, c >> 2};
+      I3 nb = mk(base[0] + bit[0], base[1] + bit[1], base[2] + bit[2]); float ww = 1;
+      for (int axis = 0; axis < 3; axis++) if (L.f[axis] > 1) ww *= bit[axis] == 1 ? t[axis] : 1 - t[axis];
+      use[c] = bit[0] < L.f[0] && bit[1] < L.f[1] && bit[2] < L.f[2] && nb.y < n.y;
+      if (nb.y < 0) nb.y = 0;
+      nb.x = m.perx ? pmod(nb.x, n.x) : iclamp(nb.x, 0, n.x - 1); nb.z = pmod(nb.z, n.z);
+      const int j = use[c] ? (int)index3(nb, n) : 0;
+      wc[c] = ww; fc[c] = m.vec[at(m, l + 1, 3, j)]; vc[c] = m.vec[at(m, l + 1, cf, j)];
+    }
+    if (on) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) if (use[c] && fc[c] > .5f && wc[c] != 0) add += wc[c] * vc[c];
+    }
+  }
   m.vec[at(m, l, ff, i)] += add;
 }
 __global__ void __launch_bounds__(256) k_mg_prolong(Ctx s, MGDev m, int l, int ff, int cf, int phase) {
