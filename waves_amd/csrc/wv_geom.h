@@ -67,9 +67,15 @@ __device__ __forceinline__ F3 coastal_velocity(const Ctx& s, F3 world, float tim
   return mkf(first * coshf(k * y) * cosf(phase) + second * coshf(2 * k * y) * cosf(2 * phase),
              first * sinhf(k * y) * sinf(phase) + second * sinhf(2 * k * y) * sinf(2 * phase), 0);
 }
+// One component of coastal_velocity, the same expression evaluated for that
+// component only
 __device__ __forceinline__ float coastal_velocity_axis(const Ctx& s, F3 world, float time, int axis) {
   if (axis == 2) return 0.0f;
-  F3 v = coastal_velocity(s, world, time); return axis == 0 ? v.x : v.y;
+  float a = ramped_amplitude(s, time), k = s.wave[1], w = s.wave[2], h = s.wave[3];
+  float sh = sinhf(k * h), phase = k * (world.x - s.forc[0]) - w * time, y = world.y + h;
+  float first = a * w / sh, second = s.stokes2 > .5f ? .75f * a * a * w * k / powf(sh, 4.0f) : 0;
+  if (axis == 0) return first * coshf(k * y) * cosf(phase) + second * coshf(2 * k * y) * cosf(2 * phase);
+  return first * sinhf(k * y) * sinf(phase) + second * sinhf(2 * k * y) * sinf(2 * phase);
 }
 __device__ __forceinline__ F3 coastal_face_position(const Ctx& s, I3 q, int axis) { F3 p = center(s, q); addc(p, axis, -(.5f * s.h[axis])); return p; }
 __device__ __forceinline__ float4 coastal_plane(const Ctx& s, I3 q, float time) {
