@@ -139,7 +139,14 @@ __device__ __forceinline__ bool solid(const Ctx& s, I3 q) {
 }
 __device__ __forceinline__ bool wet(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && s.geo[index3(q, dims(s))].y > 0.5f; }
 __device__ __forceinline__ float phi(const Ctx& s, I3 q) { q = wrap_cell(s, q); return outside(s, q) ? center(s, q).y - eta(s, q) : s.geo[index3(q, dims(s))].x; }
-__device__ __forceinline__ float press(const Ctx& s, I3 q) { q = wrap_cell(s, q); return wet(s, q) ? getv(s, candidate(s), P_(s) + index3(q, dims(s))) : 0.0f; }
+// wet(s, q) ? pressure : 0, the flag and the pressure loaded together
+__device__ __forceinline__ float press(const Ctx& s, I3 q) {
+  q = wrap_cell(s, q);
+  const bool out = outside(s, q);
+  const int64_t i = out ? 0 : index3(q, dims(s));
+  const float flag = s.geo[i].y, v = getv(s, candidate(s), P_(s) + i);
+  return !out && flag > 0.5f ? v : 0.0f;
+}
 // The wet flag and the value are loaded together (index 0 for a cell outside
 // the grid) and the flag selects: wet(s, q) ? value : 0.
 __device__ __forceinline__ float vector_at(const Ctx& s, I3 q, int which) {
