@@ -100,11 +100,19 @@ __device__ __forceinline__ float mapply0(const MGDev& m, int field, int i) {
   if (!mactive(m, 0, i)) return m.vec[at(m, 0, field, i)];
   float sum = 0; I3 q = coord(i, mdims(m, 0));
   float here = m.vec[at(m, 0, field, i)];
+  // the six neighbours' flags and values loaded together (index 0 for a
+  // neighbour outside the grid); mactive(j) selects the value
+  float flag[6], value[6]; bool in[6];
 #pragma unroll
   for (int f6 = 0; f6 < 6; f6++) {
     int axis = f6 >> 1, side = f6 & 1; I3 nb = q; addc(nb, axis, 2 * side - 1);
-    int j = mwrap(m, 0, nb);
-    float other = mactive(m, 0, j) ? m.vec[at(m, 0, field, j)] : 0;
+    const int j = mwrap(m, 0, nb); in[f6] = j >= 0;
+    const int jj = j >= 0 ? j : 0;
+    flag[f6] = m.vec[at(m, 0, 3, jj)]; value[f6] = m.vec[at(m, 0, field, jj)];
+  }
+#pragma unroll
+  for (int f6 = 0; f6 < 6; f6++) {
+    float other = in[f6] && flag[f6] > .5f ? value[f6] : 0;
     sum += snapc<H>(m, 6 + f6, i) * (here - other);
   }
   return sum;
