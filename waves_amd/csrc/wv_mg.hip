@@ -131,11 +131,19 @@ __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, in
 template <int OP, bool H, int WIDE_WAVES>
 __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
-  __shared__ int sh_off[3 * 125], sh_delta[125];
   __shared__ float part[WIDE_WAVES][64];
-  stage_stencil(m, l, sh_off, sh_delta);
   const MGLevel& L = m.lv[l]; const int N = L.count, W = L.width;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = blockIdx.x * 64 + lane;
+  // a wave's entries are the same for all its lanes: their offsets come
+  // through scalar loads, no staging in LDS
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, row = blockIdx.x * 64 + lane;
+  const int* sh_off = m.offsets + 3 * L.off_start; const int* sh_delta = m.deltas + L.off_start;
+  // wave 0 reads its rows' epilogue inputs before the join (coarse_epilogue)
+  float e_xs = 0, e_act = 0, e_b = 0, e_dinv = 0, e_dprev = 0;
+  if (wave == 0 && row < N) {
+    e_xs = m.vec[at(m, l, source, row)]; e_act = m.vec[at(m, l, 3, row)]; e_b = m.vec[at(m, l, 0, row)];
+    if (OP != 1) e_dinv = m.vec[at(m, l, 4, row)];
+    if (OP == 0 && k != 1) e_dprev = m.vec[at(m, l, 7, row)];
+  }
   float sum = 0;
   if (row < N) {
     I3 q = coord(row, mdims(m, l));
@@ -164,7 +172,13 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
   if (wave != 0 || row >= N) return;
   float ax = part[0][lane];
   for (int w = 1; w < WIDE_WAVES; w++) ax += part[w][lane];
-  coarse_epilogue<OP>(m, l, source, k, row, ax);
+  // coarse_epilogue on the preloaded values (same expressions)
+  if (!(e_act > .5f)) ax = e_xs;
+  float r = e_b - ax;
+  if (OP == 1) { m.vec[at(m, l, 6, row)] = r; return; }
+  if (OP == 2) { m.vec[at(m, l, 3 - source, row)] = e_xs + e_dinv * r; return; }
+  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * e_dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * e_dinv * r;
+  m.vec[at(m, l, 7, row)] = d; m.vec[at(m, l, 3 - source, row)] = e_xs + d;
 }
 template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
