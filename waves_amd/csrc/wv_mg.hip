@@ -108,22 +108,37 @@ __device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off
   }
   __syncthreads();
 }
-// OP 0: Chebyshev sweep k from `source`; 1: residual of `source`; 2: Jacobi sweep from `source`
+// The row epilogue (OP 0: Chebyshev sweep k from `source`; 1: residual of
+// `source`; 2: Jacobi sweep from `source`; inactive rows are the identity,
+// pressure_multigrid.glslinc:91) in two parts: its inputs read ahead of the
+// row sum (the sweep writes them only in the row's own epilogue), then the
+// update
+struct EpiIn { float xs, act, b, dinv, dprev; };
 template <int OP>
-__device__ __forceinline__ void coarse_epilogue(const MGDev& m, int l, int source, int k, int i, float ax) {
-  float xs = m.vec[at(m, l, source, i)];
-  if (!mactive(m, l, i)) ax = xs;  // inactive rows are the identity (pressure_multigrid.glslinc:91)
-  float r = m.vec[at(m, l, 0, i)] - ax;
+__device__ __forceinline__ EpiIn epilogue_load(const MGDev& m, int l, int source, int k, int i) {
+  EpiIn e{0, 0, 0, 0, 0};
+  e.xs = m.vec[at(m, l, source, i)]; e.act = m.vec[at(m, l, 3, i)]; e.b = m.vec[at(m, l, 0, i)];
+  if (OP != 1) e.dinv = m.vec[at(m, l, 4, i)];
+  if (OP == 0 && k != 1) e.dprev = m.vec[at(m, l, 7, i)];
+  return e;
+}
+template <int OP>
+__device__ __forceinline__ void epilogue_store(const MGDev& m, int l, int source, int k, int i, float ax, const EpiIn& e) {
+  if (!(e.act > .5f)) ax = e.xs;  // mactive(m, l, i)
+  float r = e.b - ax;
   if (OP == 1) { m.vec[at(m, l, 6, i)] = r; return; }
-  if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = xs + m.vec[at(m, l, 4, i)] * r; return; }
-  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
-  m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = xs + d;
+  if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = e.xs + e.dinv * r; return; }
+  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * e.dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * e.dinv * r;
+  m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = e.xs + d;
 }
 template <int OP, bool H>
 __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
+  const bool owner = rc.valid && rc.lane == 0;
+  EpiIn e{0, 0, 0, 0, 0};
+  if (owner) e = epilogue_load<OP>(m, l, source, k, rc.row);
   float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
-  if (!rc.valid || rc.lane != 0) return;
-  coarse_epilogue<OP>(m, l, source, k, rc.row, ax);
+  if (!owner) return;
+  epilogue_store<OP>(m, l, source, k, rc.row, ax, e);
 }
 // Wide rows: the WIDE_WAVES waves of a block share 64 consecutive rows, wave
 // w summing entries w, w + WIDE_WAVES, ... with every load 64 rows wide (256
@@ -137,13 +152,9 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
   // through scalar loads, no staging in LDS
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, row = blockIdx.x * 64 + lane;
   const int* sh_off = m.offsets + 3 * L.off_start; const int* sh_delta = m.deltas + L.off_start;
-  // wave 0 reads its rows' epilogue inputs before the join (coarse_epilogue)
-  float e_xs = 0, e_act = 0, e_b = 0, e_dinv = 0, e_dprev = 0;
-  if (wave == 0 && row < N) {
-    e_xs = m.vec[at(m, l, source, row)]; e_act = m.vec[at(m, l, 3, row)]; e_b = m.vec[at(m, l, 0, row)];
-    if (OP != 1) e_dinv = m.vec[at(m, l, 4, row)];
-    if (OP == 0 && k != 1) e_dprev = m.vec[at(m, l, 7, row)];
-  }
+  // wave 0 reads its rows' epilogue inputs before the join
+  EpiIn e{0, 0, 0, 0, 0};
+  if (wave == 0 && row < N) e = epilogue_load<OP>(m, l, source, k, row);
   float sum = 0;
   if (row < N) {
     I3 q = coord(row, mdims(m, l));
@@ -172,13 +183,7 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
   if (wave != 0 || row >= N) return;
   float ax = part[0][lane];
   for (int w = 1; w < WIDE_WAVES; w++) ax += part[w][lane];
-  // coarse_epilogue on the preloaded values (same expressions)
-  if (!(e_act > .5f)) ax = e_xs;
-  float r = e_b - ax;
-  if (OP == 1) { m.vec[at(m, l, 6, row)] = r; return; }
-  if (OP == 2) { m.vec[at(m, l, 3 - source, row)] = e_xs + e_dinv * r; return; }
-  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * e_dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * e_dinv * r;
-  m.vec[at(m, l, 7, row)] = d; m.vec[at(m, l, 3 - source, row)] = e_xs + d;
+  epilogue_store<OP>(m, l, source, k, row, ax, e);
 }
 template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
