@@ -846,7 +846,7 @@ __global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
 // it needs itself (the same operations on the same values, so the same bits
 // as the stored factor), which leaves one barrier per step; row k - 1 of the
 // inverse of the factor, built the same way, shares step k.
-constexpr int CHOL_THREADS = 256;
+constexpr int CHOL_THREADS = 1024;  // sixteen waves: a step's rows spread four to a wave
 __global__ void __launch_bounds__(CHOL_THREADS) k_mg_cholesky(Ctx s, MGDev m, int l) {
   if (!build_on(s, m, false)) return;
   // dense: the working matrix (lower triangle); lf: the factor; acc: rows of
@@ -877,7 +877,7 @@ __global__ void __launch_bounds__(CHOL_THREADS) k_mg_cholesky(Ctx s, MGDev m, in
       const int c = k + 1 + lane;
       if (c < N) {
         const float lc = dense[c * N + k] / lkk;
-        for (int r = c + ((wave - (c - k - 1)) & 3); r < N; r += CHOL_THREADS / 64)
+        for (int r = c + ((wave - (c - k - 1)) & (CHOL_THREADS / 64 - 1)); r < N; r += CHOL_THREADS / 64)
           dense[r * N + c] -= dense[r * N + k] / lkk * lc;
       }
     }
