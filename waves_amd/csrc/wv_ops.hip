@@ -158,13 +158,44 @@ __global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s) {
   }
 }
 // pressure_pass.glslinc:108-115 (op 16)
+// The geometry (wet flag, level set) of the cell and its six neighbours is
+// read up front (index 0 for a cell outside the grid), then coefficient() and
+// distance_face() are evaluated on it: the same values, without a load behind
+// each face's tests.
+struct CellGeo { I3 w; bool out; float2 g; };
+__device__ __forceinline__ CellGeo cell_geo(const Ctx& s, I3 x) {
+  CellGeo c; c.w = wrap_cell(s, x); c.out = outside(s, c.w);
+  c.g = s.geo[c.out ? 0 : index3(c.w, dims(s))]; return c;
+}
+__device__ __forceinline__ bool geo_wet(const CellGeo& c) { return !c.out && c.g.y > 0.5f; }
+__device__ __forceinline__ float geo_phi(const Ctx& s, const CellGeo& c) { return c.out ? center(s, c.w).y - eta(s, c.w) : c.g.x; }
 __global__ void __launch_bounds__(256) k_coefficients(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   if (s.c[4] > .5f && s.c[5] < .5f) return;
-  I3 q = coord(i, dims(s)); float diag = 0; int face = 0; bool w = wet(s, q);
-  for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
-    I3 nb = q; addc(nb, axis, sign); float k = w ? coefficient(s, q, nb, axis) : 0;
-    s.work[(int64_t)(6 + face) * s.N + i] = k; diag += k; face++;
+  I3 q = coord(i, dims(s)); float diag = 0;
+  const CellGeo cq = cell_geo(s, q); const bool w = geo_wet(cq);
+  CellGeo cn[6];
+#pragma unroll
+  for (int f6 = 0; f6 < 6; f6++) { I3 nb = q; addc(nb, f6 >> 1, (f6 & 1) ? 1 : -1); cn[f6] = cell_geo(s, nb); }
+#pragma unroll
+  for (int f6 = 0; f6 < 6; f6++) {
+    const int axis = f6 >> 1; I3 nb = q; addc(nb, axis, (f6 & 1) ? 1 : -1);
+    float k = 0;
+    // coefficient(s, q, nb, axis) with distance_face(s, q, nb, axis)
+    if (w && !solid(s, nb) && !(axis == 0 && nb.x < 0)) {
+      const float h = s.h[axis]; float d;
+      if (axis == 1 && (q.y >= s.ny || nb.y >= s.ny)) d = .5f * h;
+      else {
+        const bool wl = w, wr = geo_wet(cn[f6]);
+        if (wl && wr) d = h;
+        else {
+          const float pl = fabsf(geo_phi(s, cq)), pr = fabsf(geo_phi(s, cn[f6]));
+          d = h * gmax((wl ? pl : pr) / gmax(pl + pr, 1e-30f), 1e-6f);
+        }
+      }
+      k = 1.0f / (s.h[axis] * d);
+    }
+    s.work[(int64_t)(6 + f6) * s.N + i] = k; diag += k;
   }
   s.work[12 * s.N + i] = w ? diag : 1.0f;
 }
