@@ -169,6 +169,15 @@ __device__ __forceinline__ CellGeo cell_geo(const Ctx& s, I3 x) {
 }
 __device__ __forceinline__ bool geo_wet(const CellGeo& c) { return !c.out && c.g.y > 0.5f; }
 __device__ __forceinline__ float geo_phi(const Ctx& s, const CellGeo& c) { return c.out ? center(s, c.w).y - eta(s, c.w) : c.g.x; }
+// distance_face(s, l, r, axis) on the cells' preloaded geometry
+__device__ __forceinline__ float geo_distance_face(const Ctx& s, I3 l, I3 r, int axis, const CellGeo& cl, const CellGeo& cr) {
+  const float h = s.h[axis];
+  if (axis == 1 && (l.y >= s.ny || r.y >= s.ny)) return .5f * h;
+  const bool wl = geo_wet(cl), wr = geo_wet(cr);
+  if (wl && wr) return h;
+  const float pl = fabsf(geo_phi(s, cl)), pr = fabsf(geo_phi(s, cr));
+  return h * gmax((wl ? pl : pr) / gmax(pl + pr, 1e-30f), 1e-6f);
+}
 __global__ void __launch_bounds__(256) k_coefficients(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   if (s.c[4] > .5f && s.c[5] < .5f) return;
@@ -182,19 +191,7 @@ __global__ void __launch_bounds__(256) k_coefficients(Ctx s) {
     const int axis = f6 >> 1; I3 nb = q; addc(nb, axis, (f6 & 1) ? 1 : -1);
     float k = 0;
     // coefficient(s, q, nb, axis) with distance_face(s, q, nb, axis)
-    if (w && !solid(s, nb) && !(axis == 0 && nb.x < 0)) {
-      const float h = s.h[axis]; float d;
-      if (axis == 1 && (q.y >= s.ny || nb.y >= s.ny)) d = .5f * h;
-      else {
-        const bool wl = w, wr = geo_wet(cn[f6]);
-        if (wl && wr) d = h;
-        else {
-          const float pl = fabsf(geo_phi(s, cq)), pr = fabsf(geo_phi(s, cn[f6]));
-          d = h * gmax((wl ? pl : pr) / gmax(pl + pr, 1e-30f), 1e-6f);
-        }
-      }
-      k = 1.0f / (s.h[axis] * d);
-    }
+    if (w && !solid(s, nb) && !(axis == 0 && nb.x < 0)) k = 1.0f / (s.h[axis] * geo_distance_face(s, q, nb, axis, cq, cn[f6]));
     s.work[(int64_t)(6 + f6) * s.N + i] = k; diag += k;
   }
   s.work[12 * s.N + i] = w ? diag : 1.0f;
@@ -749,10 +746,16 @@ __global__ void __launch_bounds__(256) k_project(Ctx s) {
   I3 fn = dims(s); addc(fn, axis, 1);
   I3 r = coord(i - base, fn), l = r; addc(l, axis, -1);
   int cb = candidate(s); float dt = timestep(s);
+  // both cells' geometry and pressures read together with the face (index 0
+  // for a cell outside the grid); press() and distance_face() on them
+  const CellGeo cl = cell_geo(s, l), cr = cell_geo(s, r);
+  const float face = getv(s, cb, i);
+  const float p_l = getv(s, cb, P + (cl.out ? 0 : index3(cl.w, dims(s)))), p_r = getv(s, cb, P + (cr.out ? 0 : index3(cr.w, dims(s))));
+  const bool wl = geo_wet(cl), wr = geo_wet(cr);
   float value = 0;
-  if (!solid(s, l) && !solid(s, r) && (wet(s, l) || wet(s, r)))
-    value = getv(s, cb, i) - dt / 1000.0f * (press(s, r) - press(s, l)) / distance_face(s, l, r, axis);
-  if (axis == 0 && r.x == 0 && !solid(s, r)) value = getv(s, cb, i);
+  if (!solid(s, l) && !solid(s, r) && (wl || wr))
+    value = face - dt / 1000.0f * ((wr ? p_r : 0.0f) - (wl ? p_l : 0.0f)) / geo_distance_face(s, l, r, axis, cl, cr);
+  if (axis == 0 && r.x == 0 && !solid(s, r)) value = face;
   putv(s, cb, i, value);
 }
 // ---------------------------------------------------- extension (30 - 34) --
