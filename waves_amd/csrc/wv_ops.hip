@@ -794,8 +794,8 @@ __global__ void __launch_bounds__(256) k_ext_fill(Ctx s) {
   if (!post_ok(s)) return;
   int64_t i = GID(); if (i >= s.NF) return;
   I3 q; int axis; unpack_face(s, i, q, axis); I3 l = q; addc(l, axis, -1);
+  const float source = s.motion[source_base(s, 1) + i];  // read with the wet flags
   if (wet(s, q) || wet(s, l)) return;
-  float source = s.motion[source_base(s, 1) + i];
   float value = source < 0 ? 0 : extended_value(s, (int64_t)source, q);
   putv(s, candidate(s), U_(s) + i, value);
 }
