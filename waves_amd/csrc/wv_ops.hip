@@ -662,12 +662,21 @@ __global__ void __launch_bounds__(256) k_gravity_prep(Ctx s) {
   putv(s, cb, i, val);
 }
 // coupled_motion.glslinc:319-322 (op 47)
+// Only the duplicate faces of a periodic seam differ from their canonical face
+// (face_index): the W faces at z = nz, and the U faces at x = nx when x is
+// periodic.  One thread per duplicate face copies the canonical value, as the
+// pass over all faces does for exactly these faces.
 __global__ void __launch_bounds__(256) k_periodic_sync(Ctx s) {
   if (s.c[4] < .5f || s.c[7] > .5f) return;
-  int64_t i = GID(); if (i >= s.NF) return;
-  I3 q; int axis; unpack_face(s, i, q, axis);
-  int64_t canonical = face_index(s, q, axis);
-  if (canonical != U_(s) + i) { int cb = candidate(s); putv(s, cb, U_(s) + i, getv(s, cb, canonical)); }
+  const int64_t t = GID(), nw = (int64_t)s.nx * s.ny, nu = s.perx > .5f ? (int64_t)s.ny * s.nz : 0;
+  if (t >= nw + nu) return;
+  I3 q; int axis;
+  if (t < nw) { axis = 2; q = mk((int)(t % s.nx), (int)(t / s.nx), s.nz); }
+  else { axis = 0; const int64_t u = t - nw; q = mk(s.nx, (int)(u % s.ny), (int)(u / s.ny)); }
+  I3 n = dims(s); addc(n, axis, 1);
+  const int64_t own = (axis == 0 ? U_(s) : W_(s)) + index3(q, n);
+  const int64_t canonical = face_index(s, q, axis);
+  if (canonical != own) { int cb = candidate(s); putv(s, cb, own, getv(s, cb, canonical)); }
 }
 // coupled_motion.glslinc:282-284 (ops 63, 64, 65)
 __global__ void __launch_bounds__(256) k_defect_state(Ctx s) { float* c = s.c; if (c[57] == 1 && c[7] < .5f) { c[57] = 2; c[6] = 0; c[8] = 0; } }
@@ -967,7 +976,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 41: LRED(k_volume, 6); break;
   case 44: hipLaunchKernelGGL(k_history, dim3(1), dim3(1), 0, st, s); break;
   case 45: LRED(k_compression, 7); break;
-  case 47: L256(k_periodic_sync, s.NF); break;
+  case 47: L256(k_periodic_sync, (int64_t)s.nx * s.ny + (int64_t)s.ny * s.nz); break;  // the seams' duplicate faces
   case 48: L256(k_masks<false>, s.NF > s.N ? s.NF : s.N); break;
   case 49: L256(k_masks<true>, s.NF > s.N ? s.NF : s.N); break;
   case 55: LRED(k_ledgers, 8); break;
