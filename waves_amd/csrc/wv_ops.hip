@@ -603,16 +603,16 @@ __global__ void __launch_bounds__(256) k_gauges(Ctx s) {
     __syncthreads();
     for (int z = z0 + threadIdx.x; z < s.nz && z < z0 + 1024; z += blockDim.x) {
       float column = s.o[1];
-      // the column's cells eight at a time, loaded ahead of the in-order sum
-      for (int y0 = 0; y0 < s.ny; y0 += 8) {
-        float v[8]; bool sd[8];
+      // the column's cells 24 at a time, loaded ahead of the in-order sum
+      for (int y0 = 0; y0 < s.ny; y0 += 24) {
+        float v[24]; bool sd[24];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < 24; u++) {
           const I3 q = mk(x, y0 + u < s.ny ? y0 + u : s.ny - 1, z);
           sd[u] = solid(s, q); v[u] = getv(s, acc, index3(q, dims(s)));
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) if (y0 + u < s.ny) column += (sd[u] ? 1 : v[u]) * s.h[1];
+        for (int u = 0; u < 24; u++) if (y0 + u < s.ny) column += (sd[u] ? 1 : v[u]) * s.h[1];
       }
       columns[z - z0] = column;
     }
