@@ -91,7 +91,7 @@ __device__ __forceinline__ void queue_append(const Ctx& s, bool defer, int64_t i
   if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
 }
 __global__ void k_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
-constexpr int QUEUE_BLOCKS = 2048;
+constexpr int QUEUE_BLOCKS = 4096;
 // MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
 template <int MODE>
 __device__ __forceinline__ bool reconstruct_gate(const Ctx& s) {
