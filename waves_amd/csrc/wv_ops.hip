@@ -948,16 +948,16 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 12: LRED2(k_accept); break;
   case 14: L256(k_export_plain, s.N); break;
   case 16: L256(k_coefficients, s.N); break;
-  case 17: L256(k_reconstruct<0>, s.N); break;
+  case 17: hipLaunchKernelGGL(k_reconstruct<0>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 18:
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
     L256(k_distance_direct, s.N);
     hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s);
     break;
-  case 19: L256(k_reconstruct<2>, s.N); break;
+  case 19: hipLaunchKernelGGL(k_reconstruct<2>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 20: LRED(k_cfl, 3); break;
   case 22: L256(k_bank_copy, s.stride); break;
-  case 23: L256(k_reconstruct<1>, s.N); break;
+  case 23: hipLaunchKernelGGL(k_reconstruct<1>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 24: L256(k_flux, s.NF); break;
   case 25: LRED2(k_vof_update); break;
   case 27: case 28: {
