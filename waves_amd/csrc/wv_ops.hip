@@ -951,7 +951,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 17: hipLaunchKernelGGL(k_reconstruct<0>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 18:
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
-    L256(k_distance_direct, s.N);
+    hipLaunchKernelGGL(k_distance_direct, dim3(blocks(s.N, 64)), dim3(64), 0, st, s);  // one wave per block
     hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s);
     break;
   case 19: hipLaunchKernelGGL(k_reconstruct<2>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
@@ -964,13 +964,15 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
     const bool corr = op == 28;
     if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, corr ? 1 : 0);
-    if (corr) L256(k_maccormack_direct<true>, s.NF); else L256(k_maccormack_direct<false>, s.NF);
+    // one wave per block: waves whose faces take long paths hold no other wave's slot
+    if (corr) hipLaunchKernelGGL(k_maccormack_direct<true>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
+    else hipLaunchKernelGGL(k_maccormack_direct<false>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
     if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
     else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
     break;
   }
   case 30: L256(k_ext_seed, s.NF); break;
-  case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 256)), dim3(256), 0, st, s, (op - 31) % 2); break;
+  case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s, (op - 31) % 2); break;  // one wave per block
   case 34: L256(k_ext_fill, s.NF); break;
   case 35: LRED(k_post_courant, 5); break;
   case 41: LRED(k_volume, 6); break;
