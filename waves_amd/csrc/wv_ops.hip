@@ -987,7 +987,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 64: L256(k_defect_save, s.N); break;
   case 65: L256(k_defect_restore, s.N); break;
   case 80: LRED(k_relax_f, 10); break;
-  case 82: L256(k_relax_faces, s.NF); break;
+  case 82: hipLaunchKernelGGL(k_relax_faces, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s); break;  // one wave per block
   case 83: LRED(k_momentum, 11); break;
   case 85: hipLaunchKernelGGL(k_gauges, dim3(8), dim3(256), 0, st, s); break;
   case 86: hipLaunchKernelGGL(k_probes, dim3(1), dim3(64), 0, st, s); break;
