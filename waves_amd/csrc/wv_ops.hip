@@ -967,8 +967,8 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
     // one wave per block: waves whose faces take long paths hold no other wave's slot
     if (corr) hipLaunchKernelGGL(k_maccormack_direct<true>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
     else hipLaunchKernelGGL(k_maccormack_direct<false>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
-    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
-    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS), dim3(256), 0, st, s);
+    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s);  // one wave per block
+    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s);
     break;
   }
   case 30: L256(k_ext_seed, s.NF); break;
