@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_distance_direct(Ctx s) {
 }
 // lane k < 27 of a 32-lane group evaluates neighbour k; lane 0 gathers the 27
 // (box, candidate) pairs and runs the reference's pruned search in order
-constexpr int DIST_THREADS = 128;  // threads per block of the queued distance pass
+constexpr int DIST_THREADS = 64;  // threads per block of the queued distance pass
 __global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s) {
   if (s.c[4] > .5f && s.c[5] < .5f) return;
   __shared__ float verts[2 * VERT_FLOATS * DIST_THREADS];
