@@ -1,31 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark: accepted simulated seconds per wall second of the coupled
 free-surface step at 310x48x80 (the reference beach, water/lab_runner.gd:26,
-plain method at the reference's PCG tolerance) with the seeded 1e-3 m
-alongshore perturbation, over the 20 simulated seconds of shoaling, breaking,
-runup and backwash.
+plain FP32 method at the reference's own PCG tolerance 1e-4 and CFL 0.45) with
+the seeded 1e-3 m alongshore perturbation.
 
 A "step" is one accepted solver step (CFL substep trial accepted on the
-device).  W warmup steps run untimed from the reset state, then exactly K
-accepted steps are timed, bracketed by device synchronisation.  Batches are
-submitted ahead with one control readback per batch (water/work_scheduler.gd),
-the quota stops the device after step K.
+device).  The solver is first run untimed from rest into the breaking phase
+(t = 8 s: the waves shoal from about 4 s and break from about 6 s), then W
+warmup steps, then exactly K accepted steps are timed, bracketed by device
+synchronisation.  Submissions follow the reference's scheduler
+(water/work_scheduler.gd): at most three in flight, one control readback each,
+the quota stops the device after step K.  `value` is the timed window's
+accepted sim-s / wall-s; `full_run` is the whole 20 sim-s run from rest, timed
+in the same process.
 
-Also reported: the standalone pressure solve at 310x48x80 (seeded wavy
-surface, N(0, 0.1 m/s) faces, zero-mean RHS) in milliseconds per solve at
-relative residual 1e-6 and 1e-8, plain FP32 MGPCG against the mixed form
-(BF16 V-cycle, FP64 dots and true residual) and the mixed form with the
-learned warm start; the coupled step with the mixed PCG in the batch program;
-the refined 620x96x160 beach; rooflines of the dominant kernel (the coarse
-Chebyshev sweep at multigrid level 1) and of the MacCormack trace and the
-level-0 sweep, timed live with HIP events on the solver's stream; and the CPU
-oracle timed on the host as a baseline.
+Also reported: metric 2, the in-loop pressure time per accepted step (device
+clock around the pressure sections: hierarchy audit and rebuild, PCG,
+projection, defect corrections), median and p90, for the window and the full
+run; the standalone pressure solve at 310x48x80 at relative residual 1e-6 and
+1e-8 (plain FP32 MGPCG, the mixed BF16/FP64 form, and the mixed form with the
+learned warm start); the coupled step with the mixed PCG at 1e-6; the refined
+620x96x160 beach; rooflines of the dominant kernels timed live with HIP
+events on the solver's stream, beside a 1 GiB streaming copy; the CPU oracle
+timed on the host's cores over a window of the same workload.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -35,44 +40,94 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from waves_amd.settings import benchmark_beach, default_beach  # noqa: E402
+from waves_amd import solver as S  # noqa: E402
 from waves_amd.solver import Solver, stopped  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "accepted sim-s per wall-s @310x48x80; pressure ms/step @1e-6 rel residual"
+WINDOW_START = 8.0      # sim-s, breaking phase
+FULL_RUN = 20.0         # sim-s
 
 
-def run_steps(solver: Solver, count: int, ahead: int = 2, batches_per_submit: int = 1):
-    """Advance `count` more accepted steps; returns (control, batches submitted)."""
-    c = solver.poll(wait=True)
-    target = int(c[2]) + count
-    solver.request(target)
-    submitted = 0
-    inflight = 0
+def run_to_time(solver: Solver, t_end: float, max_batches: int = 8):
+    """Advance until the accepted time reaches t_end (request_time quota)."""
+    solver.request_time(t_end)
     while True:
-        while inflight < ahead:
-            solver.frame(batches_per_submit)
-            submitted += batches_per_submit
-            inflight += 1
-        c = solver.poll(wait=True)  # waits for the newest readback
-        inflight = 0
-        if c[2] >= target or stopped(c):
-            return c, submitted
+        c = solver.poll(wait=False)
+        st = solver.schedule_stats()
+        c = solver._last_control() if c is None else c
+        if c[1] >= t_end or stopped(c):
+            if st["inflight"] == 0:
+                return c
+            solver.poll(wait=2)
+            continue
+        if st["inflight"] < S.MAX_INFLIGHT:
+            solver.frame(max_batches)
+        else:
+            solver.poll(wait=2)
 
 
-def cpu_baseline(sample_steps: int = 1):
-    """The CPU oracle (a scalar restatement, one host thread) on the same
-    workload: reset, then `sample_steps` accepted steps."""
+def pressure_ms(rows: np.ndarray):
+    if len(rows) == 0:
+        return None
+    ms = rows[:, 0] / 1e6
+    allms = rows[:, 1] / 1e6
+    return {"median_ms": float(np.median(ms)), "p90_ms": float(np.percentile(ms, 90)), "mean_ms": float(ms.mean()),
+            "with_rejected_trials_median_ms": float(np.median(allms)),
+            "with_rejected_trials_p90_ms": float(np.percentile(allms, 90)), "steps": int(len(rows))}
+
+
+def host_info():
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count() or 1
+    return nproc, model
+
+
+def cpu_baseline(state, settings, steps: int, threads: int, model: str):
+    """The CPU oracle (C restatement, OpenMP over `threads` host threads) on
+    the benchmark window: the device's state at the window start loaded into
+    the oracle (its multigrid rebuilt from the loaded operator), then `steps`
+    accepted steps timed."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle.oracle import Oracle
-    s = benchmark_beach()
-    o = Oracle(s)
-    o.initial()
+    o = Oracle(settings)
+    for f, v in state.items():
+        o.set(f, v)
     t0 = time.perf_counter()
-    for _ in range(sample_steps):
+    c0 = o.control.copy()
+    for _ in range(steps):
         o.step()
     wall = time.perf_counter() - t0
     c = o.control
-    return {"value": float(c[1]) / wall, "unit": "accepted sim-s per wall-s", "cores": 1, "kind": "port",
-            "sample": f"oracle, {sample_steps} accepted step(s) from reset at 310x48x80 ({float(c[1]):.4f} sim-s in {wall:.1f} s)"}
+    sim = float(c[1] - c0[1])
+    o.close()
+    return {"value": sim / wall, "unit": "accepted sim-s per wall-s", "cores": threads, "kind": "port",
+            "cpu_model": model,
+            "sample": f"CPU restatement (not lavapipe): oracle/ in C with OpenMP, {threads} threads on {model}; "
+                      f"{steps} accepted steps from the window start t={float(c0[1]):.3f} s of the same 310x48x80 "
+                      f"run ({sim:.5f} sim-s in {wall:.1f} s wall)"}
+
+
+def capture_state(g: Solver):
+    from oracle import oracle as O
+    fields = (O.FIELD_BANK_A, O.FIELD_BANK_B, O.FIELD_WORK, O.FIELD_GEO, O.FIELD_CONTROL, O.FIELD_PLANE,
+              O.FIELD_MOTION, O.FIELD_HISTORY, O.FIELD_COASTAL_HISTORY, O.FIELD_IMG_PLANES, O.FIELD_IMG_KINDS,
+              O.FIELD_PROBE, O.FIELD_MG_VECTORS, O.FIELD_SNAPSHOT, O.FIELD_PREVIOUS)
+    st = {f: g.read(f) for f in fields}
+    audit = g.read(S.FIELD_AUDIT, np.uint32)
+    audit[0] = 0      # the oracle keeps its coarse operator in its own layout: rebuild it
+    st[O.FIELD_AUDIT] = audit
+    return st
 
 
 def pressure_leg(repeats: int = 3, n=(310, 48, 80), spacing=(0.025, 0.025, 0.05), learned: bool = True):
@@ -117,78 +172,109 @@ def pressure_leg(repeats: int = 3, n=(310, 48, 80), spacing=(0.025, 0.025, 0.05)
     return out
 
 
-def inloop_leg(steps: int = 600):
-    """The coupled step with the batch program's PCG in plain FP32 and in the
-    mixed form (wv_set_precision), same configuration and start: wall ms per
-    accepted step and the acceptance record of both."""
-    out = {}
+def inloop_leg(steps: int = 600, tol: float = 1e-6):
+    """The coupled step at PCG tolerance `tol` with the batch program's PCG in
+    plain FP32 and in the mixed form (wv_set_precision), same start: wall ms
+    per accepted step, in-loop pressure ms per step and the acceptance record."""
+    out = {"tolerance": tol, "steps": steps}
     for prec, name in ((0, "plain_fp32"), (1, "mixed_bf16_fp64")):
-        g = Solver(benchmark_beach())
+        g = Solver(benchmark_beach(pressure_tolerance=tol))
         g.start()
         g.set_precision(prec)
-        run_steps(g, 20)
-        g.synchronize()
+        g.run_steps(20)
         c0 = g.poll(wait=True).copy()
         t0 = time.perf_counter()
-        c, _ = run_steps(g, steps)
+        c = g.run_steps(steps)
         g.synchronize()
         wall = time.perf_counter() - t0
         c = g.poll(wait=True)
-        out[name] = {"ms_per_step": wall / steps * 1e3, "sim_seconds": float(c[1] - c0[1]), "rejections": float(c[3] - c0[3]),
-                     "pcg_iterations": float(c[18] - c0[18]), "failure": int(c[7])}
+        rows = g.step_times()[int(c0[2]):]
+        done = int(c[2] - c0[2])
+        out[name] = {"ms_per_step": wall / max(done, 1) * 1e3, "steps": done, "sim_seconds": float(c[1] - c0[1]),
+                     "rejections": float(c[3] - c0[3]), "pcg_iterations": float(c[18] - c0[18]),
+                     "failure": int(c[7]), "pressure": pressure_ms(rows)}
         g.close()
-    out["identical_acceptance"] = (out["plain_fp32"]["sim_seconds"] == out["mixed_bf16_fp64"]["sim_seconds"] and
-                                   out["plain_fp32"]["rejections"] == out["mixed_bf16_fp64"]["rejections"])
+    a, b = out["plain_fp32"], out["mixed_bf16_fp64"]
+    out["identical_acceptance"] = a["sim_seconds"] == b["sim_seconds"] and a["rejections"] == b["rejections"]
     return out
 
 
-def refined_leg(steps: int = 40):
+def refined_leg(steps: int = 20):
     """The refined beach 620x96x160 (9,523,200 cells): wall ms per accepted
     coupled step from the start, and the standalone pressure solve forms."""
-    st = benchmark_beach(n=(620, 96, 160), spacing=(0.0125, 0.0125, 0.025))
+    st = benchmark_beach(n=(620, 96, 160), spacing=(0.0125, 0.0125, 0.025), history_capacity=256)
     g = Solver(st)
     g.start()
-    run_steps(g, 5)
-    g.synchronize()
+    g.run_steps(5)
     t0 = time.perf_counter()
-    c, _ = run_steps(g, steps)
+    c = g.run_steps(steps)
     g.synchronize()
     wall = time.perf_counter() - t0
-    c = g.poll(wait=True)
     out = {"coupled_ms_per_step": wall / steps * 1e3, "steps": steps, "sim_time": float(c[1]), "failure": int(c[7])}
     g.close()
     out["pressure"] = pressure_leg(repeats=1, n=st.n, spacing=st.spacing, learned=False)
     return out
 
 
+def full_run(t_end: float = FULL_RUN):
+    """The whole 20 sim-s run from rest, timed (reset excluded)."""
+    s = benchmark_beach()
+    g = Solver(s)
+    g.start()
+    t0 = time.perf_counter()
+    c = run_to_time(g, t_end)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    c = g.poll(wait=True)
+    st = g.schedule_stats()
+    rows = g.step_times()
+    out = {"value": float(c[1]) / wall, "unit": "accepted sim-s per wall-s", "sim_seconds": float(c[1]),
+           "wall_s": wall, "steps": int(c[2]), "ms_per_step": wall / max(int(c[2]), 1) * 1e3,
+           "rejections": int(c[3]), "pcg_iterations": int(c[18]), "failure": int(c[7]),
+           "submissions": st["submissions"], "readbacks_per_accepted_step": st["readbacks"] / max(int(c[2]), 1),
+           "gpu_s": st["gpu_ms"] / 1e3, "R_fluid": st["r_fluid"], "pressure": pressure_ms(rows)}
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--window-start", type=float, default=WINDOW_START)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-full", action="store_true", help="skip the full 20 sim-s run")
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the in-loop mixed and refined-grid legs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this solver runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
+        print(json.dumps({"error": "this solver runs on one GPU (no multi-GPU path): --gpus must be 1"}))
         sys.exit(2)
 
     s = benchmark_beach()
     solver = Solver(s)
     solver.start()
-    c, _ = run_steps(solver, args.warmup)
+    t_pre = time.perf_counter()
+    c = run_to_time(solver, args.window_start)
+    preroll_wall = time.perf_counter() - t_pre
+    cpu_state = None if args.no_cpu else capture_state(solver)
+    c = solver.run_steps(args.warmup)
     solver.synchronize()
+    c = solver.poll(wait=True)
     t_start_sim, steps_start, iters_start, rej_start = float(c[1]), int(c[2]), float(c[18]), float(c[3])
+    st0 = solver.schedule_stats()
     t0 = time.perf_counter()
-    c, batches = run_steps(solver, args.steps)
+    c = solver.run_steps(args.steps)
     solver.synchronize()
     wall = time.perf_counter() - t0
     c = solver.poll(wait=True)
+    st1 = solver.schedule_stats()
     sim = float(c[1]) - t_start_sim
     steps = int(c[2]) - steps_start
     value = sim / wall
+    window_rows = solver.step_times()[steps_start:]
 
     # rooflines, timed live with HIP events on the solver's stream.  The dominant
     # kernel by step time is the coarse Chebyshev sweep family; its heaviest
@@ -204,7 +290,8 @@ def main():
     NF = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
     lv = solver.levels()
     count1, width1 = lv[1][0] * lv[1][1] * lv[1][2], lv[1][11]
-    copy_us = solver.time_kernel(3, reps=200)
+    copy_us = solver.time_kernel(6, reps=20)
+    copy_gbs = 2 * (1 << 30) / (copy_us * 1e-6) / 1e9
     traffic_all = {}
     tfile = os.path.join(ROOT, "profiles", "traffic_kernels.json")
     if os.path.exists(tfile):
@@ -221,8 +308,10 @@ def main():
         achieved = nbytes / (us * 1e-6) / 1e9
         rooflines.append({"bound": "hbm", "kernel": label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_all.get(key, {}).get("hbm_bytes_per_launch"),
-                          "avg_us": us, "bytes_per_launch": nbytes})
-    rooflines[0]["copy_GBs"] = 8 * N / (copy_us * 1e-6) / 1e9
+                          "avg_us": us, "bytes_per_launch": nbytes, "frac_of_copy": achieved / copy_gbs})
+    rooflines[0]["copy_GBs"] = copy_gbs
+    rooflines[0]["copy"] = "streaming float4 copy, 1 GiB read + 1 GiB written"
+    done_steps = max(steps, 1)
     result = {
         "metric": METRIC,
         "value": value,
@@ -230,27 +319,37 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / max(steps, 1) * 1e3,
+        "ms_per_step": wall / done_steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: reference beach from rest, seeded 1e-3 m alongshore perturbation",
-        "config": {"workload": "coupled free-surface step, 310x48x80 beach (1,190,400 cells), PCG rel tol 1e-4, 20 sim-s run",
-                   "grid": [310, 48, 80], "sim_seconds_timed": sim, "sim_time_start": t_start_sim,
-                   "batches": batches, "pcg_iterations": float(c[18]) - iters_start,
-                   "rejections": float(c[3]) - rej_start, "failure": int(c[7])},
+        "config": {"workload": f"coupled free-surface step, 310x48x80 beach (1,190,400 cells), plain FP32, PCG rel tol "
+                               f"1e-4, CFL 0.45: {steps} accepted steps from t={t_start_sim:.3f} s (breaking phase, "
+                               f"after an untimed run from rest to {args.window_start:g} s)",
+                   "grid": [310, 48, 80], "sim_time_start": t_start_sim, "sim_seconds_timed": sim,
+                   "preroll_wall_s": preroll_wall,
+                   "pcg_iterations_per_step": (float(c[18]) - iters_start) / done_steps,
+                   "rejections": float(c[3]) - rej_start, "failure": int(c[7]),
+                   "submissions": st1["submissions"] - st0["submissions"],
+                   "readbacks_per_accepted_step": (st1["readbacks"] - st0["readbacks"]) / done_steps,
+                   "R_fluid_window": (st1["gpu_ms"] - st0["gpu_ms"]) * 1e-3 / max(sim, 1e-30)},
+        "pressure_in_loop": pressure_ms(window_rows),
         "roofline": rooflines[0],
         "rooflines": rooflines,
     }
     solver.close()
+    if not args.no_full:
+        result["full_run"] = full_run()
     if not args.no_pressure:
         result["pressure"] = pressure_leg()
     if not args.no_extra:
         result["mixed_in_loop"] = inloop_leg()
         result["refined_620x96x160"] = refined_leg()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        nproc, model = host_info()
+        result["cpu_baseline"] = cpu_baseline(cpu_state, s, args.cpu_steps, nproc, model)
     print(json.dumps(result))
 
 

@@ -66,7 +66,12 @@ enum {
   WV_FIELD_AUDIT = 15,        /* hierarchy audit state (uint32) */
   WV_FIELD_MG_ROWS = 16,      /* coarse operator rows, k-major, levels >= 1 */
   WV_FIELD_MG_LINV = 17,      /* last level: dense L^-1, row-major */
-  WV_FIELD_PRESSURE64 = 18    /* N double: pressure of the last standalone solve in FP64 */
+  WV_FIELD_PRESSURE64 = 18,   /* N double: pressure of the last standalone solve in FP64 */
+  WV_FIELD_PREVIOUS = 19,     /* 9 N float, the hierarchy audit's previous-trial operator copy */
+  WV_FIELD_STEP_TIMES = 20    /* capacity x 4 uint64 per accepted step k (row k - 1): device ns of the
+                                 accepted trial's pressure sections (hierarchy audit/rebuild, PCG, projection,
+                                 defect corrections), device ns of every trial's pressure sections since
+                                 the previous acceptance, the trial count c[32], the sections timed */
 };
 
 /* _init(settings) + start() + _initialize (water/pressure_gpu.gd:40-47, 82-127):
@@ -82,14 +87,40 @@ int wv_reset(wv_solver *s);
  * quota the device's trial start checks (pressure_pass.glslinc:20). Thread-safe. */
 int wv_request_steps(wv_solver *s, int64_t steps);
 int wv_request_time(wv_solver *s, double seconds);
-/* frame() (water/work_scheduler.gd:41-87): enqueue `batches` whole batches of
- * the coupled step program (water/coupled_gpu.gd:38-56) with the current quota,
- * followed by one asynchronous 384-byte control readback. Non-blocking. */
-int wv_advance(wv_solver *s, int batches);
+/* frame() (water/work_scheduler.gd:41-87): one non-blocking submission of
+ * whole batches of the coupled step program (water/coupled_gpu.gd:38-56,
+ * water/coastal_gpu.gd:28-33) with the current quota, followed by one
+ * asynchronous 384-byte control readback.  At most max_batches; with
+ * budget_ms > 0 only as many as the per-batch GPU cost estimate (an EMA of
+ * measured submissions, as the reference's per-chunk costs) fits in the
+ * budget, at least one (the reference: gpu_budget_ms 14); nothing while
+ * WV_MAX_INFLIGHT submissions are unfinished (max_inflight_submissions 3).
+ * Returns the number of batches enqueued (>= 0) or a negative WV_E* code. */
+#define WV_MAX_INFLIGHT 3
+int wv_advance(wv_solver *s, int max_batches, float budget_ms);
 /* buffer_get_data_async(control) callback (water/work_scheduler.gd:80-87):
- * copies the newest completed readback into out[96]; *fresh = 1 if it is newer
- * than the last poll. With wait != 0 blocks until all enqueued work is done. */
+ * retires finished submissions in order and copies the newest readback into
+ * out[96]; *fresh = 1 if a readback arrived since the last poll.  wait: 0 does
+ * not block, 1 blocks until all enqueued work is done, 2 blocks until the
+ * oldest unfinished submission is done. */
 int wv_poll_control(wv_solver *s, float *out96, int *fresh, int wait);
+/* Scheduler telemetry (water/work_scheduler.gd:18-40: gpu_ns, record_us,
+ * R_fluid, queue depth).  Counters from the last wv_reset. */
+typedef struct wv_schedule_stats {
+  int64_t submissions;      /* wv_advance calls that enqueued work, one control readback each */
+  int64_t readbacks;        /* readbacks retired by wv_poll_control / wv_advance */
+  int64_t batches;          /* batches enqueued */
+  int inflight;             /* submissions not yet retired */
+  int max_inflight;         /* deepest queue seen (maximum_queue) */
+  int64_t inflight_batches; /* batches of the unretired submissions */
+  double gpu_ms;            /* device time of the retired submissions (total_gpu_ns / 1e6) */
+  double record_us;         /* host time spent enqueueing (total_record_us) */
+  double ema_batch_ms;      /* per-batch cost estimate */
+  double accepted_time;     /* c[1] of the newest readback */
+  double accepted_steps;    /* c[2] of the newest readback */
+  double r_fluid;           /* GPU seconds per accepted simulated second, -1 before time advances */
+} wv_schedule_stats;
+int wv_get_schedule_stats(wv_solver *s, wv_schedule_stats *out);
 /* parity / render export I/O (no reference counterpart: the reference's
  * buffers are GPU-only; the planes/kinds export replaces the Texture3DRD
  * upload of water/pressure_gpu.gd:106-115). Synchronous. */
@@ -108,12 +139,19 @@ const char *wv_last_error(wv_solver *s);
 /* MG level descriptors: 16 ints per level, the reference's `meta` layout
  * (water/pressure_multigrid.gd:54, 61-68). Returns the level count. */
 int wv_levels(wv_solver *s, int *meta16, int max_levels);
+/* The same level plan computed on the host alone, without a device or a
+ * handle: multigrid.initialize (water/pressure_multigrid.gd:33-69) is CPU
+ * planning in the reference too.  Returns the level count, WV_EPLAN when the
+ * grid cannot be planned. */
+int wv_plan_levels(const wv_settings *settings, int *meta16, int max_levels);
 /* Average device duration [us] of one launch of a hot kernel, timed with HIP
  * events on the solver's stream over `reps` launches on the current state
  * (kernel: 0 = level-0 Chebyshev sweep, 1 = fine masked 7-point SpMV,
  * 2 = level-0 residual, 3 = plain copy of N floats, the bandwidth reference,
  * 4 = Chebyshev sweep on MG level 1, 5 = MacCormack forward trace, op 27;
- * kernels 4 and 5 run against a copy of the control vector with a trial open).
+ * kernels 4 and 5 run against a copy of the control vector with a trial open;
+ * 6 = streaming copy of 1 GiB into another 1 GiB, two buffers allocated for
+ * the measurement and freed after it: the large-working-set bandwidth reference).
  * Writes only scratch that every trial recomputes before use (V-cycle / PCG /
  * Galerkin vectors, the MacCormack forward field and trace cache): the
  * accepted state is unchanged. */

@@ -58,6 +58,7 @@ static void *field_ptr(WO *s, int field, int64_t *count, int *bytes) {
   case 15: *count = s->state_count; return s->state;
   case 16: *count = 2 * s->row_count; return s->rows;
   case 17: { int64_t n = s->meta[16 * (s->L - 1) + 8]; *count = n * n; return s->matrix + s->meta[16 * (s->L - 1) + 9]; }
+  case 19: *count = 9 * s->N; return s->previous;
   }
   *count = 0; return NULL;
 }

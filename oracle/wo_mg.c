@@ -144,7 +144,9 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
   if (op == 11 || op == 12) { coarse_op(s, op, l); return; }
   int N = mcount(s, l); I3 nl = mdims(s, l);
   switch (op) {
-  case 1: /* pressure_multigrid.glslinc:139-150 */
+  case 1: { /* pressure_multigrid.glslinc:139-150 */
+    uint32_t changed = 0;
+    WO_PARALLEL_COUNT(changed)
     for (int i = 0; i < N; i++) {
       float value = 0;
       if (l == 0) value = s->geo[2 * i + 1] > .5f ? 1 : 0;
@@ -154,12 +156,14 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
         for (int z = 0; z < f.v[2]; z++) for (int y = 0; y < f.v[1]; y++) for (int x = 0; x < f.v[0]; x++)
           if (active_cell(s, l - 1, mwrap(s, l - 1, i3(base.v[0] + x, base.v[1] + y, base.v[2] + z)))) value = 1;
       }
-      if (s->vec[at(s, l, 3, i)] != value) s->state[20 + l]++;
+      if (s->vec[at(s, l, 3, i)] != value) changed++;
       s->vec[at(s, l, 3, i)] = value;
     }
+    s->state[20 + l] += changed;
     return;
+  }
   case 16:
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       int ids[8]; float w[8]; int size = compute_transfers(s, l, i, ids, w);
       for (int k = 0; k < md(s, l, 14); k++) {
         float *e = s->transfer + 2 * ((int64_t)md(s, l, 13) + (int64_t)k * N + i);
@@ -191,7 +195,7 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
   }
   case 13: {
     int width = md(s, l, 11);
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       I3 q = coord(i, nl);
       for (int k = 0; k < width; k++) {
         I3 off = moffset(s, l, k); int j = mwrap(s, l, i3(q.v[0] + off.v[0], q.v[1] + off.v[1], q.v[2] + off.v[2]));
@@ -202,7 +206,7 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
     return;
   }
   case 4: /* pressure_multigrid.glslinc:183-189 */
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       float sum = 0; I3 q = coord(i, nl);
       if (l == 0) {
         sum = fine_coefficient(s, 12, i);
@@ -217,22 +221,22 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
       s->vec[at(s, l, 4, i)] = 1 / sum;
     }
     return;
-  case 5: for (int i = 0; i < N; i++) { s->vec[at(s, l, 0, i)] = s->work[i]; s->vec[at(s, l, 1, i)] = 0; s->vec[at(s, l, 2, i)] = 0; } return;
+  case 5: WO_PARALLEL for (int i = 0; i < N; i++) { s->vec[at(s, l, 0, i)] = s->work[i]; s->vec[at(s, l, 1, i)] = 0; s->vec[at(s, l, 2, i)] = 0; } return;
   case 15: { /* pressure_multigrid.glslinc:192-196 */
     int source = field % 4, k = field / 4;
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       float r = s->vec[at(s, l, 0, i)] - apply(s, l, source, i);
       float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * s->vec[at(s, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * s->vec[at(s, l, 4, i)] * r;
       s->vec[at(s, l, 7, i)] = d; s->vec[at(s, l, 3 - source, i)] = s->vec[at(s, l, source, i)] + d;
     }
     return;
   }
-  case 14: for (int i = 0; i < N; i++) s->vec[at(s, l, 2, i)] = s->vec[at(s, l, 4, i)] * s->vec[at(s, l, 0, i)]; return;
-  case 6: { int source = field, dest = 3 - source; for (int i = 0; i < N; i++) s->vec[at(s, l, dest, i)] = s->vec[at(s, l, source, i)] + s->vec[at(s, l, 4, i)] * (s->vec[at(s, l, 0, i)] - apply(s, l, source, i)); return; }
-  case 7: for (int i = 0; i < N; i++) s->vec[at(s, l, 6, i)] = s->vec[at(s, l, 0, i)] - apply(s, l, field, i); return;
+  case 14: WO_PARALLEL for (int i = 0; i < N; i++) s->vec[at(s, l, 2, i)] = s->vec[at(s, l, 4, i)] * s->vec[at(s, l, 0, i)]; return;
+  case 6: { int source = field, dest = 3 - source; WO_PARALLEL for (int i = 0; i < N; i++) s->vec[at(s, l, dest, i)] = s->vec[at(s, l, source, i)] + s->vec[at(s, l, 4, i)] * (s->vec[at(s, l, 0, i)] - apply(s, l, source, i)); return; }
+  case 7: WO_PARALLEL for (int i = 0; i < N; i++) s->vec[at(s, l, 6, i)] = s->vec[at(s, l, 0, i)] - apply(s, l, field, i); return;
   case 8: { /* pressure_multigrid.glslinc:200-224 */
     int parent = l - 1; I3 f = i3(md(s, parent, 5), md(s, parent, 6), md(s, parent, 7)), nf = mdims(s, parent);
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       float sum = 0; I3 qc = coord(i, nl);
       if (md(s, parent, 15) != 0) {
         if (active_cell(s, l, i)) for (int z = 0; z < f.v[2]; z++) for (int y = 0; y < f.v[1]; y++) for (int x = 0; x < f.v[0]; x++) {
@@ -269,7 +273,7 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
   }
   case 9: {
     int ff = field % 4, cf = field / 4;
-    for (int i = 0; i < N; i++) {
+    WO_PARALLEL for (int i = 0; i < N; i++) {
       float add = 0;
       if (active_cell(s, l, i)) { int ids[8]; float ws[8]; int size = transfers(s, l, i, ids, ws); for (int k = 0; k < size; k++) add += ws[k] * s->vec[at(s, l + 1, cf, ids[k])]; }
       s->vec[at(s, l, ff, i)] += add;
@@ -278,7 +282,7 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
   }
   case 10: { /* pressure_multigrid.glslinc:230-234 */
     int64_t G = ((int64_t)N + 127) / 128;
-    for (int64_t g = 0; g < G; g++) {
+    WO_PARALLEL for (int64_t g = 0; g < G; g++) {
       F4 lane[128];
       for (int lid = 0; lid < 128; lid++) {
         int64_t i = g * 128 + lid; F4 value = f4(0, 0, 0, 0);
@@ -349,7 +353,7 @@ void wo_audit_op(WO *s, int op, int64_t groups) {
     return;
   }
   if (op == 3) {
-    for (int64_t i = 0; i < n; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < n; i++) {
       for (int k = 0; k < 7; k++) { float value = s->work[(6 + k) * n + i]; prev[k * n + i] = value; if (st[8] != 0) cached[k * n + i] = value; }
       prev[7 * n + i] = s->geo[2 * i + 1]; prev[8 * n + i] = (float)signature(s, i);
       if (st[8] != 0) { cached[7 * n + i] = prev[7 * n + i]; cached[8 * n + i] = prev[8 * n + i]; }

@@ -212,7 +212,7 @@ static float extended_value(const WO *s, int64_t source, I3 target) {
 }
 
 #define GROUP_REDUCE(mode, ...)                                   \
-  for (int64_t g = 0; g < groups; g++) {                           \
+  WO_PARALLEL for (int64_t g = 0; g < groups; g++) {                           \
     F4 lane[128];                                                  \
     for (int lid = 0; lid < 128; lid++) { int64_t i = g * 128 + lid; __VA_ARGS__; lane[lid] = v; } \
     s->partial[g] = tree_reduce(lane, mode);                       \
@@ -226,7 +226,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   int acc = accepted(s), cand = candidate(s);
   if (op == 40) { /* coupled_motion.glslinc:234-270 */
     float shape = PC_FIX(s, 0);
-    for (int64_t i = 0; i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < total; i++) {
       if (i < N) {
         I3 q = coord(i, n); F3 lo, hi; for (int k = 0; k < 3; k++) { lo.v[k] = (float)q.v[k] * PC_H(s, k); hi.v[k] = lo.v[k] + PC_H(s, k); }
         float f = get_value(s, acc, i);
@@ -298,7 +298,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   if (op == 56) { if (!setup) return; F4 v = global_reduce(s, 0); c[39] += v.v[0]; c[65] += v.v[1]; c[66] += v.v[2]; return; }
   if (op == 47) {
     if (c[4] < .5f || c[7] > .5f) return;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis); int64_t canonical = face_index(s, q, axis);
       if (canonical != U_(s) + i) put_value(s, cand, U_(s) + i, get_value(s, cand, canonical));
     }
@@ -306,7 +306,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   }
   if (op == 48 || op == 49) {
     int initial = op == 49; if (!initial && (c[4] < .5f || c[6] < .5f || c[7] > .5f)) return;
-    for (int64_t i = 0; i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < total; i++) {
       if (i < N) {
         float flag = wet(s, coord(i, n)) ? 1 : 0;
         if (initial) { s->motion[pressure_mask_base(s, 0) + i] = flag; s->motion[pressure_mask_base(s, 1) + i] = flag; }
@@ -363,17 +363,21 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   }
   if (op == 22) {
     if (!setup) return;
-    for (int64_t i = 0; i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < total; i++) {
       if (i < s->stride) put_value(s, cand, i, get_value(s, acc, i));
       if (i < N) s->motion[N + i] = get_value(s, acc, i) > .5f ? 1 : 0;
     }
     return;
   }
-  if (op == 23) { if (setup) for (int64_t i = 0; i < N && i < total; i++) { F4 e = wo_transport_plane(s, coord(i, n)); memcpy(s->plane + 4 * i, e.v, 16); } return; }
+  if (op == 23) {
+    if (!setup) return;
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) { F4 e = wo_transport_plane(s, coord(i, n)); memcpy(s->plane + 4 * i, e.v, 16); }
+    return;
+  }
   if (op == 24) {
     if (!setup) return;
     int sweep_axis = ((int)c[2] + (int)c[21]) % 3;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis);
       float amount = 0;
       if (axis == sweep_axis && !wall_face(s, q, axis)) {
@@ -422,7 +426,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   if (op == 27 || op == 28) {
     if (!setup) return;
     int flags = (int)PC_NUM(s, 3);
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis);
       F3 x; for (int k = 0; k < 3; k++) x.v[k] = ((float)q.v[k] + .5f) * PC_H(s, k);
       x.v[axis] -= .5f * PC_H(s, axis);
@@ -444,7 +448,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   }
   if (op == 30) {
     if (c[4] < .5f || c[6] < .5f || c[7] > .5f) return;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis); I3 l = q; l.v[axis]--;
       s->motion[source_base(s, 0) + i] = !wall_face(s, q, axis) && (wet(s, q) || wet(s, l)) ? (float)i : -1;
     }
@@ -453,7 +457,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   if (op >= 31 && op <= 33) {
     if (c[4] < .5f || c[6] < .5f || c[7] > .5f) return;
     int rb = (op - 31) % 2, wb = 1 - rb;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis); float source = s->motion[source_base(s, rb) + i], best = 1e20f;
       if (!wall_face(s, q, axis)) {
         if (source >= 0) { I3 sq; int sa; unpack_face(s, (int64_t)source, &sq, &sa); F3 d = periodic_delta(s, q, sq); best = dot3(d, d); }
@@ -471,7 +475,7 @@ void wo_motion_op(WO *s, int op, int64_t groups) {
   }
   if (op == 34) {
     if (c[4] < .5f || c[6] < .5f || c[7] > .5f) return;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis); I3 l = q; l.v[axis]--;
       if (wet(s, q) || wet(s, l)) continue;
       float source = s->motion[source_base(s, 1) + i];
@@ -523,7 +527,7 @@ void wo_coastal_op(WO *s, int op, int64_t groups) {
   if (op == 81) { if (!setup) return; F4 ch = global_reduce(s, 0); c[64] += ch.v[0]; for (int axis = 0; axis < 3; axis++) c[67 + axis] += ch.v[axis + 1]; return; }
   if (op == 82) {
     if (!setup) return;
-    for (int64_t i = 0; i < NF && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(NF, total); i++) {
       I3 q; int axis; unpack_face(s, i, &q, &axis);
       F3 position = wo_coastal_face_position(s, q, axis); float old = get_value(s, cand, U_(s) + i), value = old;
       if (!wall_face(s, q, axis)) {

@@ -3,7 +3,7 @@
 #include "wo_state.h"
 
 /* pressure_core.glslinc:135 sum_partial / 139-143 pressure_partial */
-#define FOR_GROUPS(s, groups) for (int64_t g = 0; g < (groups); g++)
+#define FOR_GROUPS(s, groups) WO_PARALLEL for (int64_t g = 0; g < (groups); g++)
 
 static void pcg_iteration_op(WO *s, int op, int64_t groups) {
   I3 n = dims(s); float dt = timestep(s); int64_t N = s->N;
@@ -36,7 +36,7 @@ static void pcg_iteration_op(WO *s, int op, int64_t groups) {
   }
   if (op == 5) {
     if (!solving(s)) return;
-    for (int64_t i = 0; i < N && i < groups * 128; i++) { I3 q = coord(i, n); s->work[3 * N + i] = wet(s, q) ? operator_at(s, q, 1) : 0; }
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, groups * 128); i++) { I3 q = coord(i, n); s->work[3 * N + i] = wet(s, q) ? operator_at(s, q, 1) : 0; }
     return;
   }
   if (op == 6) {
@@ -84,7 +84,7 @@ static void pcg_iteration_op(WO *s, int op, int64_t groups) {
   }
   if (op == 10) {
     if (!solving(s)) return;
-    for (int64_t i = 0; i < N && i < groups * 128; i++) s->work[2 * N + i] = s->work[N + i] + s->c[12] * s->work[2 * N + i];
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, groups * 128); i++) s->work[2 * N + i] = s->work[N + i] + s->c[12] * s->work[2 * N + i];
     return;
   }
 }
@@ -105,7 +105,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   I3 n = dims(s); int64_t N = s->N; float dt = timestep(s);
   int64_t total = groups * 128;
   if (op == 0) { /* pressure_pass.glslinc:3-19 */
-    for (int64_t i = 0; i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < total; i++) {
       if (i < s->stride) { s->a[i] = 0; s->b[i] = 0; }
       if (i < 96) s->c[i] = 0;
       if (i < N) {
@@ -119,7 +119,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   if (op == 90) { /* seeded alongshore perturbation of the still surface */
     float amp = s->opts[6]; uint32_t seed = (uint32_t)s->opts[7];
     if (amp == 0) return;
-    for (int64_t i = 0; i < N && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) {
       I3 q = coord(i, n); if (solid(s, q)) continue;
       float ph = center(s, q).v[1] - (eta(s, q) + amp * wo_perturb_unit(seed, q.v[2]));
       float f = clampf(0.5f - ph / PC_H(s, 1), 0.0f, 1.0f); s->a[i] = f; s->b[i] = f;
@@ -136,7 +136,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   if (op == 2) { /* pressure_pass.glslinc:25-50 */
     if (s->c[5] < 0.5f) return;
     int acc = accepted(s), cand = candidate(s); int64_t P = P_(s), Ub = U_(s), Vb = V_(s), Wb = W_(s);
-    for (int64_t i = 0; i < total && i < s->stride; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(total, s->stride); i++) {
       float val = get_value(s, cand, i);
       if (i < N) s->work[13 * N + i] = 0;
       if (i >= P) val = wet(s, coord(i - P, n)) ? get_value(s, acc, i) : 0;
@@ -154,7 +154,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   if (op == 11) { /* pressure_pass.glslinc:52-62 */
     if (s->c[4] < 0.5f || s->c[6] < 0.5f || s->c[7] > 0.5f) return;
     int cand = candidate(s); int64_t P = P_(s), Ub = U_(s), Vb = V_(s), Wb = W_(s);
-    for (int64_t i = Ub; i < total && i < P; i++) {
+    WO_PARALLEL for (int64_t i = Ub; i < wo_min64(total, P); i++) {
       int axis = i < Vb ? 0 : (i < Wb ? 1 : 2); int64_t base = axis == 0 ? Ub : (axis == 1 ? Vb : Wb); I3 fn = n; fn.v[axis]++;
       I3 r = coord(i - base, fn), l = r; l.v[axis]--;
       float value = 0;
@@ -211,7 +211,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
     return;
   }
   if (op == 14) { /* pressure_pass.glslinc:104-107 */
-    for (int64_t i = 0; i < N && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) {
       float f = get_value(s, accepted(s), i);
       s->img_planes[4 * i] = 0; s->img_planes[4 * i + 1] = 1; s->img_planes[4 * i + 2] = 0; s->img_planes[4 * i + 3] = f;
       s->img_kinds[i] = f == 0 ? 0 : (f == 1 ? 1 : 2);
@@ -220,7 +220,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   }
   if (op == 16) { /* pressure_pass.glslinc:108-115 */
     if (s->c[4] > .5f && s->c[5] < .5f) return;
-    for (int64_t i = 0; i < N && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) {
       I3 q = coord(i, n); float diag = 0; int face = 0; int w = wet(s, q);
       for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
         I3 nb = q; nb.v[axis] += sign; float k = w ? coefficient(s, q, nb, axis) : 0;
@@ -232,12 +232,12 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   }
   if (op == 17) { /* pressure_pass.glslinc:116-126 */
     if (s->c[4] > .5f && s->c[5] < .5f) return;
-    for (int64_t i = 0; i < N && i < total; i++) { F4 e = wo_reconstruct_pressure(s, coord(i, n)); memcpy(s->plane + 4 * i, e.v, 16); }
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) { F4 e = wo_reconstruct_pressure(s, coord(i, n)); memcpy(s->plane + 4 * i, e.v, 16); }
     return;
   }
   if (op == 19) { /* pressure_pass.glslinc:127-130 */
     if (s->c[4] > .5f) return;
-    for (int64_t i = 0; i < N && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) {
       I3 q = coord(i, n); float f = get_value(s, accepted(s), i);
       F4 e = wo_reconstruct_pressure(s, q); memcpy(s->img_planes + 4 * i, e.v, 16);
       s->img_kinds[i] = f <= 0 ? 0 : (f >= 1 ? 1 : 2);
@@ -246,7 +246,7 @@ void wo_pressure_op(WO *s, int op, int64_t groups) {
   }
   if (op == 18) { /* pressure_pass.glslinc:131-137 */
     if (s->c[4] > .5f && s->c[5] < .5f) return;
-    for (int64_t i = 0; i < N && i < total; i++) {
+    WO_PARALLEL for (int64_t i = 0; i < wo_min64(N, total); i++) {
       I3 q = coord(i, n); float ph = wo_geometry_distance(s, q);
       s->geo[2 * i] = ph; s->geo[2 * i + 1] = solid(s, q) ? -1.0f : (ph < 0 ? 1.0f : 0.0f);
     }

@@ -13,6 +13,21 @@ typedef struct { float v[3]; } F3;
 typedef struct { float v[4]; } F4;
 
 #define NCTRL 96
+
+/* The invocations of one op write disjoint outputs (as on the GPU), so an
+ * op's invocation loop may run on several host threads (OpenMP); the
+ * per-group reduction partials and the serial finalisers keep their order,
+ * so the results do not depend on the thread count.  Ops whose serial loop
+ * accumulates into shared state (the multigrid scatter) stay serial. */
+#ifdef _OPENMP
+#define WO_PARALLEL _Pragma("omp parallel for schedule(dynamic, 512)")
+#define WO_PARALLEL_COUNT(x) _Pragma(WO_STR(omp parallel for schedule(static) reduction(+:x)))
+#define WO_STR(x) #x
+#else
+#define WO_PARALLEL
+#define WO_PARALLEL_COUNT(x)
+#endif
+static inline int64_t wo_min64(int64_t a, int64_t b) { return a < b ? a : b; }
 #define MAXL 12
 
 struct WO {

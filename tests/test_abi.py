@@ -54,7 +54,7 @@ def test_invalid_arguments_are_rejected():
     assert lib.wv_create(None, ctypes.byref(h)) == -1
     st = S._Settings()  # zero grid
     assert lib.wv_create(ctypes.byref(st), ctypes.byref(h)) == -1
-    assert lib.wv_advance(None, 1) == -1
+    assert lib.wv_advance(None, 1, ctypes.c_float(0)) == -1
     assert lib.wv_request_time(None, 1.0) == -1
 
 
@@ -69,6 +69,33 @@ def test_push_constant_block_matches_reference_layout():
     np.testing.assert_allclose(pc[20:24], [0.08, dispersion_k(1.4, 0.45), 2 * np.pi / 1.4, 0.45], rtol=1e-6)
     assert pc[24] == np.float32(1e-4) and pc[26] == 1 and pc[27] == 6  # valid band + transport cache
     np.testing.assert_allclose(pc[28:32], [-1.75, 2.5, 12.0, 4.0])
+
+
+def test_host_level_plan_matches_oracle_at_every_config_size():
+    """wv_plan_levels (no device) gives the oracle's 16-int level descriptors
+    (water/pressure_multigrid.gd:33-69) for the narrow flume, the beach and the
+    refined beach, with the level counts of the derived-size table (5, 7, 8)."""
+    from oracle.oracle import lib as olib, _fp
+    from waves_amd.settings import narrow_flume
+    cases = ((narrow_flume(), 5), (default_beach(), 7),
+             (default_beach(n=(620, 96, 160), spacing=(0.0125, 0.0125, 0.025)), 8),
+             (narrow_flume(n=(64, 48, 4)), None), (narrow_flume(n=(64, 48, 4), periodic_x=True), None))
+    for s, count in cases:
+        g = S.plan_levels(s)
+        L = olib()
+        h = L.wo_create(_fp(s.push_constants()), _fp(s.options())) if s.cells < 2_000_000 else None
+        if h:
+            o = []
+            for l in range(L.wo_levels(h)):
+                m = (ctypes.c_int * 16)()
+                L.wo_level_meta(h, l, m)
+                o.append(list(m))
+            L.wo_destroy(h)
+            assert g == o, (s.n, g, o)
+        if count is not None:
+            assert len(g) == count, (s.n, len(g))
+    coarse = [m[8] for m in S.plan_levels(default_beach())]
+    assert coarse == [1190400, 297600, 37440, 4680, 600, 100, 50]
 
 
 def test_telemetry_decode_masks_recoverable_failures():

@@ -50,6 +50,11 @@ struct Ctx {
   int* mq;             // faces whose MacCormack trace needs the general sampling path (NF)
   unsigned* mq_count;  // queue lengths: [0] forward trace (op 27), [1] correction (op 28)
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
+  // pressure-section timing on the device clock (telemetry, no reference counterpart)
+  unsigned long long* tstate;      // [0] section start, [1] section active, [2] trial ns, [3] ns since last acceptance,
+                                   // [4] accepted steps seen, [5] trial count seen, [6] sections of the trial
+  unsigned long long* step_times;  // per accepted step (history capacity rows): trial ns, all-trials ns, trials, sections
+  float tick_ns;                   // device wall-clock period [ns]
 };
 
 struct I3 { int x, y, z; };

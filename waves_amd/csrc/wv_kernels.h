@@ -17,12 +17,14 @@ struct MGLevel {
   int margin[3];   // largest |offset| per axis of the stencil: rows this far from every edge are interior
   int lanes;       // lanes cooperating on one row in the coarse-level sweeps (power of two <= 32)
   int rlanes;      // lanes cooperating on one node in the restriction into this level (power of two <= 32)
+  int ga;          // lanes per fine row in the Galerkin stage 1 out of this level (power of two <= 16)
   int box[3];      // extent of one fine row's (A P) box over coarse nodes (Galerkin stage 1)
   int wrap1;       // the stencil reaches at most one period along z (and x when periodic):
                    // a neighbour index is row + delta with one +-period correction
 };
 struct MGDev {
   int L, perx;
+  int tail;                  // first level run by the one-workgroup tail kernel, -1: none
   MGLevel lv[WV_MAXL];
   const int* offsets;        // stencil offsets, 3 ints each
   const signed char* slots;  // per level: 125 delta slots -> stencil index, -1 if absent
@@ -41,11 +43,18 @@ struct MGDev {
 };
 
 int wv_launch_op(const Ctx& s, int op, hipStream_t st);
+// pressure-section timing: begin / end stamps around a batch's pressure
+// section, and the per-accepted-step record after the acceptance op
+void wv_launch_press_begin(const Ctx& s, hipStream_t st);
+void wv_launch_press_end(const Ctx& s, hipStream_t st);
+void wv_launch_step_record(const Ctx& s, hipStream_t st);
 void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st);
 // Multigrid: build = hierarchy audit + Galerkin rebuild (phase 2);
 // cycle = one V-cycle (phase 0 initial / 1 iteration) ending in the fused
 // z write-back, partials and PCG finaliser (op 4 or op 9).
 void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st);
+// first level of the one-workgroup tail for this hierarchy on the current device (-1: none)
+int wv_mg_tail_level(const MGDev& m);
 void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st);
 void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st);
 void wv_launch_spmv_timed(const Ctx& s, hipStream_t st);

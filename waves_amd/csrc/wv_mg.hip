@@ -50,11 +50,21 @@ __device__ __forceinline__ float group_sum(float v, int G) {
   for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-// sum_k A[row, k] v_field[col_k]; every lane returns the row total
+// When G lanes share a row, the lanes' products go to the row's LDS slice
+// (entry k at prod[k * R + r], R rows per block) and lane 0 of the row adds
+// them in k order -- the order of the reference's loop
+// (pressure_multigrid.glslinc:98-104) -- so the row sum is bit-identical to
+// one thread walking the row.  A G > 1 call must be made by every thread of
+// the block (it contains barriers); G is uniform over a launch.
+#define MG_ROW_PROD_MAX 64   // narrow rows (k_mg_coarse): at most 64 entries
+// sum_k A[row, k] v_field[col_k]; lane 0 of the row returns the row total in
+// stencil order (every lane when G == 1)
 template <bool H>
-__device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
+__device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, const RowCtx& rc, const int* sh_off, const int* sh_delta, float* prod) {
   const MGLevel& L = m.lv[l]; int N = L.count, W = L.width;
+  const int R = (int)blockDim.x / rc.G, r = (int)threadIdx.x / rc.G;
   float sum = 0;
+  auto take = [&](int k, float p) { if (rc.G == 1) sum += p; else prod[k * R + r] = p; };
   if (rc.valid) {
     I3 q = coord(rc.row, mdims(m, l));
     bool interior = q.x >= L.margin[0] && q.x < L.n[0] - L.margin[0] && q.y >= L.margin[1] && q.y < L.n[1] - L.margin[1] &&
@@ -62,8 +72,8 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
     const float* row = m.rows + L.row_off + rc.row;
     const uint16_t* row16 = m.rows16 + L.row_off + rc.row;
     const float* v = m.vec + L.vec_off + (int64_t)field * N;
-    // unrolled so several coefficient and neighbour loads are in flight; the
-    // additions stay in k order
+    auto coef = [&](int k) { return H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]; };
+    // unrolled so several coefficient and neighbour loads are in flight
     if (L.wrap1) {
       // every row on one path: row + delta, one period added or removed where
       // the offset crosses the periodic z (x) edge.  Entries towards the
@@ -76,7 +86,7 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
         j += zz < 0 ? N : (zz >= nz ? -N : 0);
         if (m.perx) { int xx = q.x + sh_off[3 * k]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
         j = min(max(j, 0), N - 1);
-        return (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[j];
+        return coef(k) * v[j];
       };
       if (rc.G == 1) {
         // one row per thread (the large coarse levels): a unit stride lets the
@@ -85,20 +95,24 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
         for (int k = 0; k < W; k++) sum += entry(k);
       } else {
 #pragma unroll 8
-        for (int k = rc.lane; k < W; k += rc.G) sum += entry(k);
+        for (int k = rc.lane; k < W; k += rc.G) take(k, entry(k));
       }
     } else if (interior) {
 #pragma unroll 8
-      for (int k = rc.lane; k < W; k += rc.G) sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[rc.row + sh_delta[k]];
+      for (int k = rc.lane; k < W; k += rc.G) take(k, coef(k) * v[rc.row + sh_delta[k]]);
     } else {
 #pragma unroll 4
       for (int k = rc.lane; k < W; k += rc.G) {
         int j = mwrap(m, l, mk(q.x + sh_off[3 * k], q.y + sh_off[3 * k + 1], q.z + sh_off[3 * k + 2]));
-        sum += (H ? bf16_to_f32(row16[(int64_t)k * N]) : row[(int64_t)k * N]) * v[j >= 0 ? j : 0];
+        take(k, coef(k) * v[j >= 0 ? j : 0]);
       }
     }
   }
-  return group_sum(sum, rc.G);
+  if (rc.G == 1) return sum;
+  __syncthreads();
+  if (rc.valid && rc.lane == 0) for (int k = 0; k < W; k++) sum += prod[k * R + r];
+  __syncthreads();
+  return sum;
 }
 __device__ __forceinline__ void stage_stencil(const MGDev& m, int l, int* sh_off, int* sh_delta) {
   const MGLevel& L = m.lv[l];
@@ -132,21 +146,24 @@ __device__ __forceinline__ void epilogue_store(const MGDev& m, int l, int source
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = e.xs + d;
 }
 template <int OP, bool H>
-__device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta) {
+__device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta, float* prod) {
   const bool owner = rc.valid && rc.lane == 0;
   EpiIn e{0, 0, 0, 0, 0};
   if (owner) e = epilogue_load<OP>(m, l, source, k, rc.row);
-  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta);
+  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta, prod);
   if (!owner) return;
   epilogue_store<OP>(m, l, source, k, rc.row, ax, e);
 }
 // Wide rows: the WIDE_WAVES waves of a block share 64 consecutive rows, wave
-// w summing entries w, w + WIDE_WAVES, ... with every load 64 rows wide (256
-// contiguous bytes); the partials join in wave order in LDS.
+// w taking entries w, w + WIDE_WAVES, ... with every load 64 rows wide (256
+// contiguous bytes); each product goes to LDS (entry e of the block's rows at
+// prod[e][lane]) and wave 0 adds a row's products in entry order, the order
+// of the reference's loop (pressure_multigrid.glslinc:98-104).
+#define MG_WIDE_MAX 128      // entries of a wide row (the 125-point Galerkin stencil)
 template <int OP, bool H, int WIDE_WAVES>
 __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
-  __shared__ float part[WIDE_WAVES][64];
+  __shared__ float prod[MG_WIDE_MAX][64];
   const MGLevel& L = m.lv[l]; const int N = L.count, W = L.width;
   // a wave's entries are the same for all its lanes: their offsets come
   // through scalar loads, no staging in LDS
@@ -155,7 +172,6 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
   // wave 0 reads its rows' epilogue inputs before the join
   EpiIn e{0, 0, 0, 0, 0};
   if (wave == 0 && row < N) e = epilogue_load<OP>(m, l, source, k, row);
-  float sum = 0;
   if (row < N) {
     I3 q = coord(row, mdims(m, l));
     const float* A = m.rows + L.row_off + row;
@@ -169,36 +185,47 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
         j += zz < 0 ? N : (zz >= nz ? -N : 0);
         if (m.perx) { int xx = q.x + sh_off[3 * e]; j += xx < 0 ? nx : (xx >= nx ? -nx : 0); }
         j = min(max(j, 0), N - 1);
-        sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j];
+        prod[e][lane] = (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j];
       }
     } else {
       for (int e = wave; e < W; e += WIDE_WAVES) {
         int j = mwrap(m, l, mk(q.x + sh_off[3 * e], q.y + sh_off[3 * e + 1], q.z + sh_off[3 * e + 2]));
-        sum += (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j >= 0 ? j : 0];
+        prod[e][lane] = (H ? bf16_to_f32(A16[(int64_t)e * N]) : A[(int64_t)e * N]) * v[j >= 0 ? j : 0];
       }
     }
   }
-  part[wave][lane] = sum;
   __syncthreads();
   if (wave != 0 || row >= N) return;
-  float ax = part[0][lane];
-  for (int w = 1; w < WIDE_WAVES; w++) ax += part[w][lane];
+  float ax = 0;
+#pragma unroll 8
+  for (int e = 0; e < W; e++) ax += prod[e][lane];
   epilogue_store<OP>(m, l, source, k, row, ax, e);
 }
 template <int OP, bool H>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ int sh_off[3 * 125], sh_delta[125];
+  extern __shared__ float prod[];   // (256 / G) rows x W entries when G > 1
   stage_stencil(m, l, sh_off, sh_delta);
   const MGLevel& L = m.lv[l];
   RowCtx rc = row_ctx(L.lanes, L.count);
-  coarse_row<OP, H>(m, l, source, k, rc, sh_off, sh_delta);
+  coarse_row<OP, H>(m, l, source, k, rc, sh_off, sh_delta, prod);
 }
-// op 8 into level l (from l-1) with G lanes per coarse node over the fine support
+// op 8 into level l (from l-1) with G lanes per coarse node over the fine
+// support.  With G > 1 the lanes' weighted children go to the node's LDS slice
+// (child t at part[t * R + r], R nodes per block, with an active flag) and
+// lane 0 adds the active ones in child order, as the reference's loop does
+// (pressure_multigrid.glslinc:200-224), so the sum is bit-identical to one
+// thread walking the support.
+#define MG_RESTRICT_MAX 64   // children of a node: a 4x4x4 (trilinear) or 2x2x2 (aggregation) box
 __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
-  if (!mg_enabled(s, phase)) return;
+  if (!mg_enabled(s, phase)) return;   // uniform over the launch: no thread reaches a barrier
+  extern __shared__ float part[];      // (256 / G) nodes x 64 children when G > 1, then their flags
   const MGLevel& C = m.lv[l]; int parent = l - 1; const MGLevel& P = m.lv[parent];
   RowCtx rc = row_ctx(C.rlanes, C.count);
+  const int R = 256 / rc.G, r = (int)threadIdx.x / rc.G;
+  unsigned char* on_flag = reinterpret_cast<unsigned char*>(part + (rc.G > 1 ? R * MG_RESTRICT_MAX : 0));
+  int total = 0;
   float sum = 0;
   I3 nf = mdims(m, parent), nl = mdims(m, l);
   if (rc.valid && mactive(m, l, rc.row)) {
@@ -210,7 +237,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
       lo[a]--; ext[a] = 4;
       if ((a == 2 || (a == 0 && m.perx)) && nfa[a] <= 4) { lo[a] = 0; ext[a] = nfa[a]; }
     }
-    int total = ext[0] * ext[1] * ext[2];
+    total = ext[0] * ext[1] * ext[2];
     // the flag and value loads of a child do not wait on its flag (in-range
     // index for an outside child, which adds nothing), so the loads of
     // several children are in flight together
@@ -235,12 +262,21 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
           weight *= w;
         }
       }
-      if (on) sum += weight * v;
+      if (rc.G == 1) { if (on) sum += weight * v; }
+      else { part[t * R + r] = weight * v; on_flag[t * R + r] = on ? 1 : 0; }
     }
   }
-  sum = group_sum(sum, rc.G);
+  if (rc.G > 1) {
+    __syncthreads();
+    if (rc.valid && rc.lane == 0)
+      for (int t = 0; t < total; t++) if (on_flag[t * R + r]) sum += part[t * R + r];
+  }
   if (!rc.valid || rc.lane != 0) return;
   m.vec[at(m, l, 0, rc.row)] = sum; m.vec[at(m, l, 1, rc.row)] = 0; m.vec[at(m, l, 2, rc.row)] = 0;
+}
+static size_t restrict_lds(const MGDev& m, int l) {
+  const int G = m.lv[l].rlanes;
+  return G > 1 ? (sizeof(float) + 1) * (256 / G) * MG_RESTRICT_MAX : 0;
 }
 
 // terminal smoother when coarsening stops above 64 nodes
@@ -906,8 +942,7 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
     size_t lds = sizeof(float) * m.lv[l + 1].width * GAL_THREADS;
     int G = m.lv[l + 1].rlanes > 8 ? 8 : m.lv[l + 1].rlanes;
     const size_t ap_lds = sizeof(float) * 128 * m.lv[l].box[0] * m.lv[l].box[1] * m.lv[l].box[2];
-    int ga = 1;  // lanes per fine row: about 2^18 lanes, at most 16 and the row width
-    while (ga < 16 && (int64_t)m.lv[l].count * ga < (1 << 18) && 2 * ga <= m.lv[l].width) ga *= 2;
+    const int ga = m.lv[l].ga;  // lanes per fine row (planned with the sweeps' lane target)
     hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks((int64_t)m.lv[l].count * ga, 128)), dim3(128), ap_lds, st, s, m, l, ga);
     // wide coarse stencils (125 entries: a block's 128 lanes) gather per entry;
     // narrow ones split each node's fine support over lanes with LDS slices
@@ -965,6 +1000,8 @@ static int tail_start(const MGDev& m) {
   }
   return best;
 }
+int wv_mg_tail_level(const MGDev& m) { return tail_start(m); }
+
 // dst[e] = src(e) for e < n over the workgroup, eight elements per thread in
 // flight (all loads of a round before its stores)
 template <class Src>
@@ -1148,6 +1185,12 @@ static int tail_field(int smoothing) {
   return smoothing % 2 ? 3 - down : down;
 }
 
+// dynamic LDS of k_mg_coarse: the rows' product slices when G > 1 lanes share a row
+static size_t coarse_lds(const MGDev& m, int l) {
+  const MGLevel& L = m.lv[l];
+  return L.lanes > 1 ? sizeof(float) * (256 / L.lanes) * L.width : 0;
+}
+
 template <bool H>
 struct CycleBuilder {
   const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
@@ -1156,14 +1199,14 @@ struct CycleBuilder {
     for (int sweep = 0; sweep < smoothing; sweep++) {
       if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, sweep + 1, phase);
-      else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
+      else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, field, sweep + 1, phase);
       field = 3 - field;
     }
     return field;
   }
   // water/pressure_multigrid.gd:141-154
   int cycle(int l) {
-    if (l > 0 && l == tail_start(m)) {
+    if (l > 0 && l == m.tail) {
       TailLayout tl = tail_layout(m, l);
       hipLaunchKernelGGL(k_mg_tail<H>, dim3(1), dim3(TAIL_THREADS), tl.floats * sizeof(float), st, s, m, tl, phase, smoothing);
       return tail_field(smoothing);
@@ -1173,7 +1216,7 @@ struct CycleBuilder {
       int tf = 1;
       for (int sweep = 0; sweep < 32; sweep++) {
         if (sweep == 0) hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, 1, phase);
-        else hipLaunchKernelGGL((k_mg_coarse<2, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, tf, 0, phase);
+        else hipLaunchKernelGGL((k_mg_coarse<2, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, tf, 0, phase);
         tf = 3 - tf;
       }
       return tf;
@@ -1181,8 +1224,8 @@ struct CycleBuilder {
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
     else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, 0, phase);
-    else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), 0, st, s, m, l, field, 0, phase);
-    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), 0, st, s, m, l + 1, phase);
+    else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, field, 0, phase);
+    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), restrict_lds(m, l + 1), st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
     return smooth(l, field);
@@ -1221,10 +1264,30 @@ __global__ void __launch_bounds__(256) k_mg_residual_timed(Ctx s, MGDev m) {
 __global__ void __launch_bounds__(256) k_copy_timed(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
   int64_t i = GID(); if (i < n) dst[i] = src[i];
 }
+// streaming copy of n floats, 16 B per lane, grid-stride: the large-working-set
+// bandwidth reference (one warm-up launch, then `reps` timed with events)
+__global__ void __launch_bounds__(256) k_copy_stream(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n4) {
+  for (int64_t i = GID(); i < n4; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+int wv_time_copy(float* a, float* b, int64_t n, int reps, hipStream_t st, float* ms) {
+  const int64_t n4 = n / 4;
+  const unsigned grid = 256 * 64;   // 64 blocks per CU
+  hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)b, n4);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return -1; }
+  hipEventRecord(e0, st);
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)b, n4);
+  hipEventRecord(e1, st);
+  hipError_t e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  return e == hipSuccess ? 0 : -1;
+}
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
   if (kernel == 4) {   // Chebyshev sweep 2 on level 1 (the largest coarse level), gates open
     const int l = m.L > 1 ? 1 : 0;
-    hipLaunchKernelGGL((k_mg_coarse<0, false>), dim3(nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256)), dim3(256), 0, st, c, m, l, 1, 2, 1);
+    hipLaunchKernelGGL((k_mg_coarse<0, false>), dim3(nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256)), dim3(256), coarse_lds(m, l), st, c, m, l, 1, 2, 1);
     return;
   }
   if (kernel == 3) { hipLaunchKernelGGL(k_copy_timed, dim3(nblocks(c.N, 256)), dim3(256), 0, st, c.work + 3 * c.N, m.ap, c.N); return; }

@@ -920,6 +920,50 @@ __global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, int final) {
   if (threadIdx.x == 0) s.c[26] = g.x;
 }
 
+// ---------------------------------------- pressure-section timing (telemetry) --
+// The in-loop pressure time of an accepted step: device wall-clock stamps
+// around the batch's pressure section (hierarchy audit and rebuild, PCG set-up,
+// iterations, projection and the defect-correction restore), summed over the
+// batches that section ran in -- a trial's main solve and its defect
+// corrections -- and recorded when the trial is accepted.  Not a reference op:
+// the reference's host measures whole submissions only
+// (water/work_scheduler.gd:18-40).  Single-thread kernels; they read c[] and
+// write only tstate / step_times.
+__global__ void __launch_bounds__(64) k_press_begin(Ctx s) {
+  if (threadIdx.x != 0) return;
+  const float* c = s.c;
+  unsigned long long* t = s.tstate;
+  if ((unsigned long long)c[32] != t[5]) { t[2] = 0; t[6] = 0; t[5] = (unsigned long long)c[32]; }
+  // the section does work when a trial is open, not failed, and its solve
+  // (main or defect correction) has not converged yet
+  const bool active = c[4] > .5f && c[7] < .5f && (c[6] < .5f || c[5] > .5f || c[57] == 2);
+  t[1] = active ? 1 : 0;
+  t[0] = wall_clock64();
+}
+__global__ void __launch_bounds__(64) k_press_end(Ctx s) {
+  if (threadIdx.x != 0) return;
+  unsigned long long* t = s.tstate;
+  if (!t[1]) return;
+  const unsigned long long dt = (unsigned long long)((double)(wall_clock64() - t[0]) * (double)s.tick_ns);
+  t[2] += dt; t[3] += dt; t[6]++; t[1] = 0;
+}
+__global__ void __launch_bounds__(64) k_step_record(Ctx s) {
+  if (threadIdx.x != 0) return;
+  const float* c = s.c;
+  unsigned long long* t = s.tstate;
+  const unsigned long long steps = (unsigned long long)c[2];
+  if (steps == t[4]) return;
+  t[4] = steps;
+  if (steps >= 1 && steps <= (unsigned long long)s.hist_cap) {
+    unsigned long long* row = s.step_times + 4 * (steps - 1);
+    row[0] = t[2]; row[1] = t[3]; row[2] = (unsigned long long)c[32]; row[3] = t[6];
+  }
+  t[3] = 0;
+}
+void wv_launch_press_begin(const Ctx& s, hipStream_t st) { hipLaunchKernelGGL(k_press_begin, dim3(1), dim3(64), 0, st, s); }
+void wv_launch_press_end(const Ctx& s, hipStream_t st) { hipLaunchKernelGGL(k_press_end, dim3(1), dim3(64), 0, st, s); }
+void wv_launch_step_record(const Ctx& s, hipStream_t st) { hipLaunchKernelGGL(k_step_record, dim3(1), dim3(64), 0, st, s); }
+
 // ------------------------------------------------------------- launchers ----
 static inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 #define L256(k, n) hipLaunchKernelGGL(k, dim3(blocks(n, 256)), dim3(256), 0, st, s)

@@ -154,6 +154,11 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     int r = 1;
     while (r < 32 && 2 * r <= box && (int64_t)L.count * r < (1 << 20)) r *= 2;
     L.rlanes = r;
+    // Galerkin stage 1 (A P rows of this level's fine rows): about the same
+    // lane target, at most 16 lanes and the row width
+    int ga = 1;
+    while (ga < 16 && (int64_t)L.count * ga < lane_target && 2 * ga <= L.width) ga *= 2;
+    L.ga = ga;
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
   p.node_count = node_count; p.row_count = row_count;
@@ -172,14 +177,19 @@ struct wv_solver {
   std::vector<void*> allocations;
   float* quota_dev = nullptr;
   float* quota_host = nullptr;   // pinned
-  float* ctrl_host = nullptr;    // pinned, readback slots
+  float* ctrl_host = nullptr;    // pinned, one 96-float readback slot per submission in flight
   float* ctl_timing = nullptr;   // device control vector used only by wv_time_kernel
+  float* ctl_saved = nullptr;    // the caller's control vector around a standalone pressure solve
   wv_net* net = nullptr;         // learned warm start (not owned)
-  hipEvent_t ctrl_event = nullptr;
+  // submissions in flight (water/work_scheduler.gd:48-54: at most three), oldest first
+  struct Submission { hipEvent_t start = nullptr, end = nullptr; int batches = 0; double record_us = 0; };
+  Submission sub[WV_MAX_INFLIGHT];
+  int sub_head = 0, sub_count = 0;
   std::atomic<double> quota{0.0};
   float last_uploaded_quota = NAN;
-  bool readback_pending = false;
   float last_control[WV_NCTRL] = {0};
+  wv_schedule_stats sched{};     // counters of the scheduler (telemetry of water/work_scheduler.gd:18-40)
+  double ema_batch_ms = 0;       // EMA of one batch's GPU time (the reference's per-chunk cost EMA)
   hipGraphExec_t batch_exec = nullptr;
   hipGraph_t batch_graph = nullptr;
   int smoothing = 3, iterations = 8;
@@ -220,7 +230,10 @@ static void release(wv_solver* s) {
   s->allocations.clear();
   if (s->quota_host) hipHostFree(s->quota_host);
   if (s->ctrl_host) hipHostFree(s->ctrl_host);
-  if (s->ctrl_event) hipEventDestroy(s->ctrl_event);
+  for (auto& b : s->sub) {
+    if (b.start) hipEventDestroy(b.start);
+    if (b.end) hipEventDestroy(b.end);
+  }
   if (s->stream) hipStreamDestroy(s->stream);
 }
 
@@ -272,13 +285,17 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.aux, 2 * c.N);
   ALLOC(c.mq, c.NF);
   ALLOC(c.mq_count, 4);
+  ALLOC(c.tstate, 8);
+  ALLOC(c.step_times, 4 * (s->history_capacity + 1));
   ALLOC(s->ctl_timing, WV_NCTRL);
+  ALLOC(s->ctl_saved, WV_NCTRL);
   ALLOC(s->quota_dev, 4);
   c.quota = s->quota_dev;
   MGDev& m = s->mg;
   memset(&m, 0, sizeof m);
   m.L = s->plan.L; m.perx = c.perx > .5f;
   for (int l = 0; l < m.L; l++) m.lv[l] = s->plan.lv[l];
+  { const char* e = getenv("WAVES_MG_TAIL"); m.tail = (e && e[0] == '0') ? -1 : wv_mg_tail_level(m); }
   int* offs; signed char* slots; int* deltas;
   ALLOC(offs, s->plan.offsets.size());
   ALLOC(deltas, s->plan.deltas.size());
@@ -300,9 +317,16 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   if (hipMemcpy(offs, s->plan.offsets.data(), s->plan.offsets.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(slots, s->plan.slots.data(), s->plan.slots.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(deltas, s->plan.deltas.data(), s->plan.deltas.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) { release(s); delete s; return WV_EHIP; }
-  if (hipHostMalloc((void**)&s->quota_host, 64) != hipSuccess || hipHostMalloc((void**)&s->ctrl_host, WV_NCTRL * sizeof(float)) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ctrl_event, hipEventDisableTiming) != hipSuccess) { release(s); delete s; return WV_EHIP; }
-  memset(s->ctrl_host, 0, WV_NCTRL * sizeof(float));
+  if (hipHostMalloc((void**)&s->quota_host, 64) != hipSuccess ||
+      hipHostMalloc((void**)&s->ctrl_host, WV_MAX_INFLIGHT * WV_NCTRL * sizeof(float)) != hipSuccess) { release(s); delete s; return WV_EHIP; }
+  for (auto& b : s->sub)
+    if (hipEventCreate(&b.start) != hipSuccess || hipEventCreate(&b.end) != hipSuccess) { release(s); delete s; return WV_EHIP; }
+  memset(s->ctrl_host, 0, WV_MAX_INFLIGHT * WV_NCTRL * sizeof(float));
+  {  // device wall clock (pressure-section timing): kHz -> ns per tick
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    c.tick_ns = (float)(1e6 / khz);
+  }
   *out = s;
   return WV_OK;
 }
@@ -321,7 +345,7 @@ static int sync_quota(wv_solver* s) {
   float q = (float)s->quota.load();
   if (q == s->last_uploaded_quota) return 0;
   HIPCHECK(s, hipStreamSynchronize(s->stream));  // the pinned slot may still be in flight
-  *s->quota_host = q;
+  s->quota_host[0] = q;
   HIPCHECK(s, hipMemcpyAsync(s->quota_dev, s->quota_host, sizeof(float), hipMemcpyHostToDevice, s->stream));
   s->last_uploaded_quota = q;
   return 0;
@@ -357,6 +381,7 @@ static int record_batch(wv_solver* s) {
   for (int op : {80, 81, 82, 83, 84, 87, 88}) d(op);
   d(17); d(18); d(16);
   d(2); d(47); d(63); d(64);
+  wv_launch_press_begin(s->ctx, s->stream);
   wv_mg_build(s->ctx, s->mg, s->stream);
   const bool mixed = s->precision == 1;
   if (mixed) wv_mg_to_bf16(s->ctx, s->mg, s->stream);
@@ -372,10 +397,14 @@ static int record_batch(wv_solver* s) {
     else wv_mg_cycle(s->ctx, s->mg, 1, s->smoothing, false, s->stream);                  // + op 9
     d(10);
   }
-  d(11); d(65); d(47);
+  d(11); d(65);
+  wv_launch_press_end(s->ctx, s->stream);
+  d(47);
   for (int op : {30, 31, 32, 33, 34, 47, 48}) d(op);
   d(35); d(36);
-  d(62); d(66); d(12); d(13); d(44); d(19);
+  d(62); d(66); d(12); d(13);
+  wv_launch_step_record(s->ctx, s->stream);
+  d(44); d(19);
   d(85); d(89);
   return 0;
 }
@@ -406,11 +435,19 @@ extern "C" int wv_reset(wv_solver* s) {
   HIPCHECK(s, hipMemsetAsync(s->mg.state, 0, (64 + 32 * s->history_capacity) * sizeof(unsigned), s->stream));
   HIPCHECK(s, hipMemsetAsync(s->ctx.history, 0, (s->history_capacity * 4 + 4) * sizeof(float4), s->stream));
   HIPCHECK(s, hipMemsetAsync(s->ctx.chist, 0, (s->history_capacity * 5 + 5) * sizeof(float4), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->ctx.tstate, 0, 8 * sizeof(unsigned long long), s->stream));
+  HIPCHECK(s, hipMemsetAsync(s->ctx.step_times, 0, 4 * (s->history_capacity + 1) * sizeof(unsigned long long), s->stream));
   for (int op : {0, 40}) dispatch(s, op);
   if (s->ctx.pert_amp != 0) dispatch(s, 90);
   for (int op : {17, 18, 16, 19, 41, 42, 49, 85, 86}) dispatch(s, op);
   HIPCHECK(s, hipGetLastError());
   HIPCHECK(s, hipStreamSynchronize(s->stream));
+  // a reset is a new solver in the reference (water/lab_runner.gd:15-31): no
+  // submission in flight, scheduler counters from zero
+  s->sub_head = s->sub_count = 0;
+  s->sched = wv_schedule_stats{};
+  s->ema_batch_ms = 0;
+  HIPCHECK(s, hipMemcpy(s->last_control, s->ctx.c, sizeof s->last_control, hipMemcpyDeviceToHost));
   return WV_OK;
 }
 
@@ -425,37 +462,99 @@ extern "C" int wv_request_time(wv_solver* s, double seconds) {
   return WV_OK;
 }
 
-extern "C" int wv_advance(wv_solver* s, int batches) {
-  if (!s || batches < 0) return WV_EINVAL;
+// Retire completed submissions, oldest first (consume_timestamps and the
+// buffer_get_data_async callback of water/work_scheduler.gd:18-40, 80-87):
+// the newest readback becomes the control copy, each submission's GPU time
+// (events around its batches and readback) feeds the per-batch cost EMA
+// (lerp 0.25 towards the new duration, as the reference's chunk costs) and
+// the GPU-time total.  wait: 0 none, 1 every submission, 2 the oldest one.
+static int retire(wv_solver* s, int wait, int* fresh) {
+  if (wait == 1) HIPCHECK(s, hipStreamSynchronize(s->stream));
+  bool first = true;
+  while (s->sub_count > 0) {
+    auto& b = s->sub[s->sub_head];
+    if (wait == 2 && first) HIPCHECK(s, hipEventSynchronize(b.end));
+    first = false;
+    hipError_t e = hipEventQuery(b.end);
+    if (e == hipErrorNotReady) break;
+    if (e != hipSuccess) return fail(s, WV_EHIP, std::string("control readback: ") + hipGetErrorString(e));
+    float ms = 0;
+    HIPCHECK(s, hipEventElapsedTime(&ms, b.start, b.end));
+    const double per = b.batches > 0 ? ms / b.batches : ms;
+    s->ema_batch_ms = s->ema_batch_ms > 0 ? s->ema_batch_ms + 0.25 * (per - s->ema_batch_ms) : per;
+    s->sched.gpu_ms += ms;
+    s->sched.readbacks++;
+    s->sched.inflight_batches -= b.batches;
+    memcpy(s->last_control, s->ctrl_host + WV_NCTRL * s->sub_head, sizeof s->last_control);
+    s->sub_head = (s->sub_head + 1) % WV_MAX_INFLIGHT;
+    s->sub_count--;
+    if (fresh) *fresh = 1;
+  }
+  s->sched.inflight = s->sub_count;
+  s->sched.ema_batch_ms = s->ema_batch_ms;
+  s->sched.accepted_time = s->last_control[1];
+  s->sched.accepted_steps = s->last_control[2];
+  s->sched.r_fluid = s->last_control[1] > 0 ? s->sched.gpu_ms * 1e-3 / s->last_control[1] : -1;
+  return 0;
+}
+
+// frame() (water/work_scheduler.gd:41-87): one submission of up to
+// max_batches whole batches, as many as the per-batch cost estimate fits in
+// budget_ms (at least one), followed by one asynchronous control readback;
+// nothing when WV_MAX_INFLIGHT submissions are unfinished.  The quota is
+// uploaded inside the submission from its own pinned slot, so no earlier
+// submission is waited for.
+extern "C" int wv_advance(wv_solver* s, int max_batches, float budget_ms) {
+  if (!s || max_batches < 0 || !(budget_ms >= 0)) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
-  int rc = sync_quota(s); if (rc) return rc;
+  int rc = retire(s, 0, nullptr); if (rc) return rc;
+  if (max_batches == 0 || s->sub_count >= WV_MAX_INFLIGHT) return 0;
+  auto t0 = std::chrono::steady_clock::now();
+  int n = max_batches;
+  if (budget_ms > 0 && s->ema_batch_ms > 0) n = std::max(1, std::min(n, (int)(budget_ms / s->ema_batch_ms)));
+  const int slot = (s->sub_head + s->sub_count) % WV_MAX_INFLIGHT;
+  auto& b = s->sub[slot];
+  const float q = (float)s->quota.load();
+  if (q != s->last_uploaded_quota) {
+    // slot 1 + k belongs to submission slot k: reused only after that submission retired
+    s->quota_host[1 + slot] = q;
+    HIPCHECK(s, hipMemcpyAsync(s->quota_dev, s->quota_host + 1 + slot, sizeof(float), hipMemcpyHostToDevice, s->stream));
+    s->last_uploaded_quota = q;
+  }
+  HIPCHECK(s, hipEventRecord(b.start, s->stream));
   if (s->use_graph) {
     rc = ensure_graph(s); if (rc) return rc;
-    for (int b = 0; b < batches; b++) HIPCHECK(s, hipGraphLaunch(s->batch_exec, s->stream));
+    for (int k = 0; k < n; k++) HIPCHECK(s, hipGraphLaunch(s->batch_exec, s->stream));
   } else {
-    for (int b = 0; b < batches; b++) record_batch(s);
+    for (int k = 0; k < n; k++) record_batch(s);
     HIPCHECK(s, hipGetLastError());
   }
-  HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, s->ctx.c, WV_NCTRL * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-  HIPCHECK(s, hipEventRecord(s->ctrl_event, s->stream));
-  s->readback_pending = true;
-  return WV_OK;
+  HIPCHECK(s, hipMemcpyAsync(s->ctrl_host + WV_NCTRL * slot, s->ctx.c, WV_NCTRL * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHECK(s, hipEventRecord(b.end, s->stream));
+  b.batches = n;
+  s->sub_count++;
+  s->sched.submissions++;
+  s->sched.batches += n;
+  s->sched.inflight = s->sub_count;
+  s->sched.inflight_batches += n;
+  s->sched.max_inflight = std::max(s->sched.max_inflight, s->sub_count);
+  s->sched.record_us += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  return n;
 }
 
 extern "C" int wv_poll_control(wv_solver* s, float* out96, int* fresh, int wait) {
-  if (!s || !out96) return WV_EINVAL;
+  if (!s || !out96 || wait < 0 || wait > 2) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
   if (fresh) *fresh = 0;
-  if (wait) HIPCHECK(s, hipStreamSynchronize(s->stream));
-  if (s->readback_pending) {
-    hipError_t e = hipEventQuery(s->ctrl_event);
-    if (e == hipSuccess) {
-      memcpy(s->last_control, s->ctrl_host, sizeof s->last_control);
-      s->readback_pending = false;
-      if (fresh) *fresh = 1;
-    } else if (e != hipErrorNotReady) return fail(s, WV_EHIP, std::string("control readback: ") + hipGetErrorString(e));
-  }
+  int rc = retire(s, wait, fresh); if (rc) return rc;
   memcpy(out96, s->last_control, sizeof s->last_control);
+  return WV_OK;
+}
+
+extern "C" int wv_get_schedule_stats(wv_solver* s, wv_schedule_stats* out) {
+  if (!s || !out) return WV_EINVAL;
+  std::lock_guard<std::mutex> g(s->lock);
+  *out = s->sched;
   return WV_OK;
 }
 
@@ -481,6 +580,8 @@ static void* field_ptr(wv_solver* s, int field, int64_t* bytes) {
   case WV_FIELD_MG_ROWS: *bytes = s->plan.row_count * 4; return m.rows;
   case WV_FIELD_MG_LINV: { int64_t n = s->plan.lv[s->plan.L - 1].count; *bytes = n * n * 4; return m.linv; }
   case WV_FIELD_PRESSURE64: *bytes = c.N * 8; return c.pacc;
+  case WV_FIELD_PREVIOUS: *bytes = 9 * c.N * 4; return m.previous;
+  case WV_FIELD_STEP_TIMES: *bytes = s->history_capacity * 4 * 8; return c.step_times;
   }
   *bytes = -1;
   return nullptr;
@@ -533,6 +634,16 @@ extern "C" int wv_synchronize(wv_solver* s) {
   return WV_OK;
 }
 
+extern "C" int wv_plan_levels(const wv_settings* settings, int* meta16, int max_levels) {
+  if (!settings || max_levels < 0) return WV_EINVAL;
+  const float* pc = settings->pc;
+  if (pc[0] < 1 || pc[1] < 1 || pc[2] < 1 || pc[4] <= 0 || pc[5] <= 0 || pc[6] <= 0) return WV_EINVAL;
+  Plan p;
+  if (!plan_levels(pc, (int)settings->opts[3], p)) return WV_EPLAN;
+  if (meta16) for (int l = 0; l < p.L && l < max_levels; l++) memcpy(meta16 + 16 * l, &p.meta[16 * l], 64);
+  return p.L;
+}
+
 extern "C" int wv_levels(wv_solver* s, int* meta16, int max_levels) {
   if (!s) return WV_EINVAL;
   int L = s->plan.L;
@@ -541,10 +652,25 @@ extern "C" int wv_levels(wv_solver* s, int* meta16, int max_levels) {
 }
 
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st);
+int wv_time_copy(float* a, float* b, int64_t n, int reps, hipStream_t st, float* ms);
 extern "C" int wv_time_kernel(wv_solver* s, int kernel, int reps, float* avg_us) {
-  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 5) return WV_EINVAL;
+  if (!s || !avg_us || reps < 1 || kernel < 0 || kernel > 6) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
   Ctx c = s->ctx;
+  if (kernel == 6) {
+    // plain copy of 2^28 floats (1 GiB read + 1 GiB written): a working set
+    // far beyond the 256 MB last-level cache, the achievable-bandwidth reference
+    const int64_t n = (int64_t)1 << 28;
+    float *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 4) != hipSuccess) return fail(s, WV_ENOMEM, "copy reference: 1 GiB");
+    if (hipMalloc(&b, n * 4) != hipSuccess) { hipFree(a); return fail(s, WV_ENOMEM, "copy reference: 1 GiB"); }
+    float ms = 0;
+    int rc = wv_time_copy(a, b, n, reps, s->stream, &ms);
+    hipFree(a); hipFree(b);
+    if (rc) return fail(s, WV_EHIP, "copy reference failed");
+    *avg_us = ms * 1000.0f / reps;
+    return WV_OK;
+  }
   if (kernel >= 4) {
     // the step kernels gate themselves on the control vector: time them against
     // a copy of it with a trial open (c[4] = c[5] = 1, no convergence, no failure)
@@ -580,9 +706,18 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
   const bool zero_mean = (flags & WV_SOLVE_ZERO_MEAN) != 0, given = (flags & WV_SOLVE_GIVEN_RHS) != 0;
   Ctx c = s->ctx; c.tol = tolerance; c.flex = mode == 2;
   const MGDev& m = s->mg; hipStream_t st = s->stream;
-  hipEvent_t e0, e1;
-  HIPCHECK(s, hipEventCreate(&e0)); HIPCHECK(s, hipEventCreate(&e1));
-  HIPCHECK(s, hipStreamSynchronize(st));
+  // finish and retire every submission first: the solve reads back through the
+  // first readback slot
+  int rc = retire(s, 1, nullptr); if (rc) return rc;
+  struct Events {  // destroyed on every return path
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() { if (a) hipEventDestroy(a); if (b) hipEventDestroy(b); }
+  } ev;
+  HIPCHECK(s, hipEventCreate(&ev.a)); HIPCHECK(s, hipEventCreate(&ev.b));
+  hipEvent_t e0 = ev.a, e1 = ev.b;
+  // the solve opens its own trial in the control vector; the caller's control
+  // vector is restored afterwards (the accepted bank is never written)
+  HIPCHECK(s, hipMemcpyAsync(s->ctl_saved, c.c, WV_NCTRL * sizeof(float), hipMemcpyDeviceToDevice, st));
   auto wall0 = std::chrono::steady_clock::now();
   HIPCHECK(s, hipEventRecord(e0, st));
   wv_solve_begin(c, !given, st);
@@ -647,7 +782,8 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
   HIPCHECK(s, hipMemcpyAsync(s->ctrl_host, c.c, sizeof ctl, hipMemcpyDeviceToHost, st));
   HIPCHECK(s, hipStreamSynchronize(st));
   memcpy(ctl, s->ctrl_host, sizeof ctl);
-  hipEventDestroy(e0); hipEventDestroy(e1);
+  HIPCHECK(s, hipMemcpyAsync(c.c, s->ctl_saved, WV_NCTRL * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHECK(s, hipStreamSynchronize(st));
   out->iterations = (int)ctl[8];
   out->converged = ctl[6] > .5f;
   out->failure = (int)ctl[7];

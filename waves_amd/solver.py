@@ -25,13 +25,15 @@ FIELD_BANK_A, FIELD_BANK_B, FIELD_WORK, FIELD_GEO, FIELD_CONTROL = 0, 1, 2, 3, 4
 FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
 FIELD_HISTORY, FIELD_COASTAL_HISTORY, FIELD_PROBE, FIELD_PARTIAL = 9, 10, 11, 12
 FIELD_MG_VECTORS, FIELD_SNAPSHOT, FIELD_AUDIT, FIELD_MG_ROWS, FIELD_MG_LINV = 13, 14, 15, 16, 17
-FIELD_PRESSURE64 = 18
+FIELD_PRESSURE64, FIELD_PREVIOUS, FIELD_STEP_TIMES = 18, 19, 20
+MAX_INFLIGHT = 3
 
 #: every entry point declared in include/waves.h
 EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_request_time",
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
-           "wv_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision", "wv_liquid_volume")
+           "wv_levels", "wv_plan_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision",
+           "wv_liquid_volume", "wv_get_schedule_stats")
 #: every entry point declared in include/waves_net.h
 NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
                "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
@@ -57,6 +59,13 @@ def stopped(c: np.ndarray) -> bool:
 
 class _Settings(ctypes.Structure):
     _fields_ = [("pc", ctypes.c_float * 32), ("opts", ctypes.c_float * 8)]
+
+
+class _ScheduleStats(ctypes.Structure):
+    _fields_ = [("submissions", ctypes.c_int64), ("readbacks", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("inflight", ctypes.c_int), ("max_inflight", ctypes.c_int), ("inflight_batches", ctypes.c_int64),
+                ("gpu_ms", ctypes.c_double), ("record_us", ctypes.c_double), ("ema_batch_ms", ctypes.c_double),
+                ("accepted_time", ctypes.c_double), ("accepted_steps", ctypes.c_double), ("r_fluid", ctypes.c_double)]
 
 
 class _SolveReport(ctypes.Structure):
@@ -85,7 +94,8 @@ def library() -> ctypes.CDLL:
         L.wv_reset.argtypes = [vp]
         L.wv_request_steps.argtypes = [vp, i64]
         L.wv_request_time.argtypes = [vp, ctypes.c_double]
-        L.wv_advance.argtypes = [vp, i32]
+        L.wv_advance.argtypes = [vp, i32, ctypes.c_float]
+        L.wv_get_schedule_stats.argtypes = [vp, ctypes.POINTER(_ScheduleStats)]
         L.wv_poll_control.argtypes = [vp, fp, ctypes.POINTER(i32), i32]
         L.wv_field_bytes.restype = i64
         L.wv_field_bytes.argtypes = [vp, i32]
@@ -97,6 +107,7 @@ def library() -> ctypes.CDLL:
         L.wv_last_error.restype = ctypes.c_char_p
         L.wv_last_error.argtypes = [vp]
         L.wv_levels.argtypes = [vp, ctypes.POINTER(i32), i32]
+        L.wv_plan_levels.argtypes = [ctypes.POINTER(_Settings), ctypes.POINTER(i32), i32]
         L.wv_time_kernel.argtypes = [vp, i32, i32, fp]
         L.wv_pressure_solve.argtypes = [vp, i32, ctypes.c_float, i32, i32, ctypes.POINTER(_SolveReport)]
         L.wv_set_precision.argtypes = [vp, i32]
@@ -121,6 +132,23 @@ def library() -> ctypes.CDLL:
         return L
 
 
+def _pack(settings: Settings) -> _Settings:
+    s = _Settings()
+    s.pc[:] = [float(v) for v in settings.push_constants()]
+    s.opts[:] = [float(v) for v in settings.options()]
+    return s
+
+
+def plan_levels(settings: Settings):
+    """The multigrid level plan (16 ints per level, the reference's `meta`,
+    water/pressure_multigrid.gd:33-69) computed on the host, no device needed."""
+    meta = (ctypes.c_int * (16 * 12))()
+    L = library().wv_plan_levels(ctypes.byref(_pack(settings)), meta, 12)
+    if L < 0:
+        raise WavesError(f"wv_plan_levels failed ({L})")
+    return [list(meta[16 * l:16 * l + 16]) for l in range(L)]
+
+
 class Solver:
     """The reference's coastal solver object (water/presentation/coastal_view.gd
     -> coastal_gpu.gd -> coupled_gpu.gd -> pressure_gpu.gd) on the HIP library."""
@@ -132,9 +160,7 @@ class Solver:
         self.ready = False
         self.last_stats: Dict = {}
         self._h = ctypes.c_void_p()
-        s = _Settings()
-        s.pc[:] = [float(v) for v in settings.push_constants()]
-        s.opts[:] = [float(v) for v in settings.options()]
+        s = _pack(settings)
         self._lib = library()
         rc = self._lib.wv_create(ctypes.byref(s), ctypes.byref(self._h))
         if rc != 0:
@@ -174,12 +200,19 @@ class Solver:
         self._check(self._lib.wv_request_time(self._h, float(seconds)))
 
     # -- scheduling: water/work_scheduler.gd:41-87 ---------------------------
-    def frame(self, batches: int = 1):
-        """Enqueue whole batches and one control readback (non-blocking)."""
-        self._check(self._lib.wv_advance(self._h, int(batches)))
+    def frame(self, max_batches: int = 1, budget_ms: float = 0.0) -> int:
+        """One submission of up to `max_batches` batches (as many as fit in
+        `budget_ms` when > 0) and one control readback, non-blocking; returns
+        the batches enqueued (0 while MAX_INFLIGHT submissions are unfinished)."""
+        n = self._lib.wv_advance(self._h, int(max_batches), float(budget_ms))
+        if n < 0:
+            self._check(n)
+        return n
 
-    def poll(self, wait: bool = False) -> Optional[np.ndarray]:
-        """Newest control readback; returns None if nothing new arrived."""
+    def poll(self, wait=False) -> Optional[np.ndarray]:
+        """Newest control readback; None if nothing new arrived (and not
+        waiting).  wait: False/0 no blocking, True/1 until all enqueued work is
+        done, 2 until the oldest unfinished submission is done."""
         out = np.zeros(96, np.float32)
         fresh = ctypes.c_int(0)
         self._check(self._lib.wv_poll_control(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(fresh), int(wait)))
@@ -187,6 +220,54 @@ class Solver:
             self.last_stats = self._stats(out)
             return out
         return None
+
+    def schedule_stats(self) -> Dict:
+        """Scheduler telemetry: submissions, readbacks, gpu_ms, record_us,
+        R_fluid (water/work_scheduler.gd:18-40)."""
+        st = _ScheduleStats()
+        self._check(self._lib.wv_get_schedule_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ScheduleStats._fields_}
+
+    def step_times(self) -> np.ndarray:
+        """(accepted steps, 4) uint64: pressure ns of the accepted trial, of all
+        trials since the previous acceptance, trial count, sections timed."""
+        a = self.read(FIELD_STEP_TIMES, np.uint64).reshape(-1, 4)
+        return a[: int(self.control[2])]
+
+    def run_steps(self, count: int, max_batches: int = 8, budget_ms: float = 0.0):
+        """Advance `count` more accepted steps with up to MAX_INFLIGHT
+        submissions in flight, sizing each submission by the observed batches
+        per accepted step so that few batches run past the quota; returns the
+        control vector when done (or stopped)."""
+        c = self.poll(wait=True)
+        target = int(c[2]) + int(count)
+        self.request(target)
+        st = self.schedule_stats()
+        b0, s0 = st["batches"], int(c[2])
+        per_step = 2.0
+        while True:
+            c = self.poll(wait=False)
+            c = self._last_control() if c is None else c
+            st = self.schedule_stats()
+            done = int(c[2])
+            if done >= target or stopped(c):
+                if st["inflight"] == 0:
+                    return c
+                self.poll(wait=2)
+                continue
+            if done > s0:
+                per_step = max(1.0, (st["batches"] - st["inflight_batches"] - b0) / (done - s0))
+            need = int(np.ceil((target - done) * per_step)) - st["inflight_batches"]
+            if need > 0 and st["inflight"] < MAX_INFLIGHT:
+                self.frame(min(max_batches, need), budget_ms)
+            else:
+                self.poll(wait=2)
+
+    def _last_control(self) -> np.ndarray:
+        out = np.zeros(96, np.float32)
+        fresh = ctypes.c_int(0)
+        self._check(self._lib.wv_poll_control(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(fresh), 0))
+        return out
 
     def synchronize(self):
         self._check(self._lib.wv_synchronize(self._h))
