@@ -49,13 +49,18 @@ WINDOW_START = 8.0      # sim-s, breaking phase
 FULL_RUN = 20.0         # sim-s
 
 
-def run_to_time(solver: Solver, t_end: float, max_batches: int = 8):
-    """Advance until the accepted time reaches t_end (request_time quota)."""
+def run_to_time(solver: Solver, t_end: float, max_batches: int = 8, curve=None):
+    """Advance until the accepted time reaches t_end (request_time quota).
+    curve: a list receiving (wall s, accepted sim-s) at each whole sim-second."""
     solver.request_time(t_end)
+    t0, mark = time.perf_counter(), 1.0
     while True:
         c = solver.poll(wait=False)
         st = solver.schedule_stats()
         c = solver._last_control() if c is None else c
+        if curve is not None and c[1] >= mark:
+            curve.append((round(time.perf_counter() - t0, 3), round(float(c[1]), 3)))
+            mark = float(np.floor(c[1])) + 1.0
         if c[1] >= t_end or stopped(c):
             if st["inflight"] == 0:
                 return c
@@ -222,7 +227,8 @@ def full_run(t_end: float = FULL_RUN):
     g = Solver(s)
     g.start()
     t0 = time.perf_counter()
-    c = run_to_time(g, t_end)
+    curve = []
+    c = run_to_time(g, t_end, curve=curve)
     g.synchronize()
     wall = time.perf_counter() - t0
     c = g.poll(wait=True)
@@ -232,7 +238,8 @@ def full_run(t_end: float = FULL_RUN):
            "wall_s": wall, "steps": int(c[2]), "ms_per_step": wall / max(int(c[2]), 1) * 1e3,
            "rejections": int(c[3]), "pcg_iterations": int(c[18]), "failure": int(c[7]),
            "submissions": st["submissions"], "readbacks_per_accepted_step": st["readbacks"] / max(int(c[2]), 1),
-           "gpu_s": st["gpu_ms"] / 1e3, "R_fluid": st["r_fluid"], "pressure": pressure_ms(rows)}
+           "gpu_s": st["gpu_ms"] / 1e3, "R_fluid": st["r_fluid"], "pressure": pressure_ms(rows),
+           "wall_at_sim_second": curve}
     g.close()
     return out
 
@@ -244,7 +251,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--window-start", type=float, default=WINDOW_START)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-full", action="store_true", help="skip the full 20 sim-s run")
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the in-loop mixed and refined-grid legs")
