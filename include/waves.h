@@ -190,6 +190,21 @@ typedef struct wv_solve_report {
  * c[17], is an FP32 sum).  Waits for the handle's stream. */
 int wv_liquid_volume(wv_solver *s, int bank, double *volume);
 
+/* Allocation-free variant of wv_liquid_volume: the same FP64 reduction (same
+ * 128-lane groups, same order, bit-identical result) of `cells` volume
+ * fractions at the device address `fraction`, times hx hy hz, written to the
+ * device address `volume` (one double), enqueued on `stream` (a hipStream_t;
+ * NULL: the null stream) and not waited for.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no allocating routine: its per-group partials live in a static
+ * device array of WV_NOALLOC_MAX_CELLS / 128 doubles (1 MiB) that each GPU
+ * holds until the process ends; larger inputs return WV_EINVAL.  Calls that
+ * overlap on one GPU share that array and race. */
+#define WV_NOALLOC_MAX_CELLS (1 << 24)
+int wv_liquid_volume_noalloc(const float *fraction, int64_t cells, float hx, float hy, float hz, double *volume,
+                             void *stream);
+
 /* Precision of the batch program's pressure solve (no reference counterpart:
  * the reference always runs mode 0).  0: the reference's FP32 PCG and
  * V-cycle; 1: the V-cycle's operators in BF16 (FP32 vectors and accumulation)

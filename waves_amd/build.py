@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwaves_hip.so")
-SOURCES = ["wv_ops.hip", "wv_mg.hip", "wv_solve.hip", "wv_runtime.hip", "wv_net.hip"]
+SOURCES = ["wv_ops.hip", "wv_mg.hip", "wv_solve.hip", "wv_runtime.hip", "wv_net.hip", "wv_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -33,7 +33,7 @@ EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_reques
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
            "wv_levels", "wv_plan_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision",
-           "wv_liquid_volume", "wv_get_schedule_stats")
+           "wv_liquid_volume", "wv_get_schedule_stats", "wv_liquid_volume_noalloc")
 #: every entry point declared in include/waves_net.h
 NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
                "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
@@ -96,6 +96,7 @@ def library() -> ctypes.CDLL:
         L.wv_request_time.argtypes = [vp, ctypes.c_double]
         L.wv_advance.argtypes = [vp, i32, ctypes.c_float]
         L.wv_get_schedule_stats.argtypes = [vp, ctypes.POINTER(_ScheduleStats)]
+        L.wv_liquid_volume_noalloc.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
         L.wv_poll_control.argtypes = [vp, fp, ctypes.POINTER(i32), i32]
         L.wv_field_bytes.restype = i64
         L.wv_field_bytes.argtypes = [vp, i32]
