@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# WAVES_ORACLE_LIB: a differently built oracle library (the mutation sweep of
+# tests/oracle_mutations.py loads deliberately broken builds through it)
+LIB = os.environ.get("WAVES_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 FIELD_BANK_A, FIELD_BANK_B, FIELD_WORK, FIELD_GEO, FIELD_CONTROL = 0, 1, 2, 3, 4
 FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
