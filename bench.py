@@ -49,17 +49,25 @@ WINDOW_START = 8.0      # sim-s, breaking phase
 FULL_RUN = 20.0         # sim-s
 
 
+PROGRESS = False
+
+
 def run_to_time(solver: Solver, t_end: float, max_batches: int = 8, curve=None):
     """Advance until the accepted time reaches t_end (request_time quota).
-    curve: a list receiving (wall s, accepted sim-s) at each whole sim-second."""
+    curve: a list receiving (wall s, accepted sim-s) at each whole sim-second;
+    with --progress each whole simulated second reached is printed to stderr."""
     solver.request_time(t_end)
     t0, mark = time.perf_counter(), 1.0
     while True:
         c = solver.poll(wait=False)
         st = solver.schedule_stats()
         c = solver._last_control() if c is None else c
-        if curve is not None and c[1] >= mark:
-            curve.append((round(time.perf_counter() - t0, 3), round(float(c[1]), 3)))
+        if c[1] >= mark:
+            if curve is not None:
+                curve.append((round(time.perf_counter() - t0, 3), round(float(c[1]), 3)))
+            if PROGRESS:
+                print(f"reached {float(c[1]):.3f} sim-s, {int(c[2])} steps, {time.perf_counter() - t0:.1f} s wall",
+                      file=sys.stderr, flush=True)
             mark = float(np.floor(c[1])) + 1.0
         if c[1] >= t_end or stopped(c):
             if st["inflight"] == 0:
@@ -255,7 +263,12 @@ def main():
     ap.add_argument("--no-full", action="store_true", help="skip the full 20 sim-s run")
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the in-loop mixed and refined-grid legs")
+    ap.add_argument("--save-state", help="write the state at the window start to this state file")
+    ap.add_argument("--load-state", help="start the window from this state file instead of the untimed run")
+    ap.add_argument("--progress", action="store_true", help="print each simulated second reached to stderr")
     args = ap.parse_args()
+    global PROGRESS
+    PROGRESS = args.progress
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU (no multi-GPU path): --gpus must be 1"}))
         sys.exit(2)
@@ -264,8 +277,14 @@ def main():
     solver = Solver(s)
     solver.start()
     t_pre = time.perf_counter()
-    c = run_to_time(solver, args.window_start)
+    if args.load_state:
+        solver.load_state(args.load_state)
+        c = solver.poll(wait=True)
+    else:
+        c = run_to_time(solver, args.window_start)
     preroll_wall = time.perf_counter() - t_pre
+    if args.save_state:
+        solver.save_state(args.save_state)
     cpu_state = None if args.no_cpu else capture_state(solver)
     c = solver.run_steps(args.warmup)
     solver.synchronize()

@@ -127,6 +127,26 @@ int wv_get_schedule_stats(wv_solver *s, wv_schedule_stats *out);
 int64_t wv_field_bytes(wv_solver *s, int field);
 int wv_read_field(wv_solver *s, int field, void *dst, int64_t bytes);
 int wv_write_field(wv_solver *s, int field, const void *src, int64_t bytes);
+/* State file (no reference counterpart: the reference's buffers are GPU-only
+ * and start cleared, water/pressure_gpu.gd:93-100; this is the identical-input
+ * harness's format).  Version WV_STATE_VERSION, little endian: a 256-byte
+ * header (magic "WAVESTAT", uint32 version, uint32 field count, int32 n[3],
+ * uint32 header bytes, the 32 + 8 setting floats of wv_settings), one 32-byte
+ * entry per field (uint32 field id, uint32 bytes per element, uint64 bytes,
+ * uint64 file offset, uint64 checksum: the sum over the bytes' little-endian
+ * 32-bit words w_i, the last zero-padded, of (w_i + 1)(2i + 1) 0x9E3779B97F4A7C15
+ * modulo 2^64), then each
+ * field's bytes at a 256-byte aligned offset.  The fields: both banks, work,
+ * geo, control, plane, motion, exported planes and kinds, both history rings,
+ * probes, multigrid vectors, snapshot, previous copy, audit state, coarse
+ * rows, coarse L^-1 and the per-step pressure timing -- everything a batch
+ * reads, so a loaded solver continues bit-identically.  wv_load_state checks
+ * magic, version, grid, settings (except the per-dispatch op and quota slots),
+ * sizes and checksums before anything reaches the device.  Both wait for the
+ * handle's stream. */
+#define WV_STATE_VERSION 1
+int wv_save_state(wv_solver *s, const char *path);
+int wv_load_state(wv_solver *s, const char *path);
 /* one reference op (water/pressure_gpu.gd:69-81 _dispatch without the MG hook),
  * for the parity harness; ops whose finaliser is fused into the previous op's
  * kernel are no-ops. Enqueued, not waited for. */

@@ -59,7 +59,7 @@ __device__ __forceinline__ float group_sum(float v, int G) {
 #define MG_ROW_PROD_MAX 64   // narrow rows (k_mg_coarse): at most 64 entries
 // sum_k A[row, k] v_field[col_k]; lane 0 of the row returns the row total in
 // stencil order (every lane when G == 1)
-template <bool H>
+template <bool H, int WC = 0>
 __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, const RowCtx& rc, const int* sh_off, const int* sh_delta, float* prod) {
   const MGLevel& L = m.lv[l]; int N = L.count, W = L.width;
   const int R = (int)blockDim.x / rc.G, r = (int)threadIdx.x / rc.G;
@@ -88,7 +88,16 @@ __device__ __forceinline__ float coarse_apply(const MGDev& m, int l, int field, 
         j = min(max(j, 0), N - 1);
         return coef(k) * v[j];
       };
-      if (rc.G == 1) {
+      if (WC > 0) {
+        // the row width known at compile time: every entry's coefficient and
+        // neighbour load is issued before the first addition (one memory
+        // round trip per row), the additions still in k order
+        float pr[WC > 0 ? WC : 1];
+#pragma unroll
+        for (int k = 0; k < WC; k++) pr[k] = entry(k);
+#pragma unroll
+        for (int k = 0; k < WC; k++) sum += pr[k];
+      } else if (rc.G == 1) {
         // one row per thread (the large coarse levels): a unit stride lets the
         // compiler keep more of the row's loads in flight
 #pragma unroll 13
@@ -145,12 +154,12 @@ __device__ __forceinline__ void epilogue_store(const MGDev& m, int l, int source
   float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * e.dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * e.dinv * r;
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = e.xs + d;
 }
-template <int OP, bool H>
+template <int OP, bool H, int WC>
 __device__ __forceinline__ void coarse_row(const MGDev& m, int l, int source, int k, const RowCtx& rc, const int* sh_off, const int* sh_delta, float* prod) {
   const bool owner = rc.valid && rc.lane == 0;
   EpiIn e{0, 0, 0, 0, 0};
   if (owner) e = epilogue_load<OP>(m, l, source, k, rc.row);
-  float ax = coarse_apply<H>(m, l, source, rc, sh_off, sh_delta, prod);
+  float ax = coarse_apply<H, WC>(m, l, source, rc, sh_off, sh_delta, prod);
   if (!owner) return;
   epilogue_store<OP>(m, l, source, k, rc.row, ax, e);
 }
@@ -201,7 +210,9 @@ __global__ void __launch_bounds__(64 * WIDE_WAVES) k_mg_coarse_wide(Ctx s, MGDev
   for (int e = 0; e < W; e++) ax += prod[e][lane];
   epilogue_store<OP>(m, l, source, k, row, ax, e);
 }
-template <int OP, bool H>
+// WC > 0: the level's width, one lane per row and the wrap-once addressing,
+// known at compile time (the 39-point level of the 310x48x80 hierarchy)
+template <int OP, bool H, int WC = 0>
 __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int source, int k, int phase) {
   if (!mg_enabled(s, phase)) return;
   __shared__ int sh_off[3 * 125], sh_delta[125];
@@ -209,7 +220,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int so
   stage_stencil(m, l, sh_off, sh_delta);
   const MGLevel& L = m.lv[l];
   RowCtx rc = row_ctx(L.lanes, L.count);
-  coarse_row<OP, H>(m, l, source, k, rc, sh_off, sh_delta, prod);
+  coarse_row<OP, H, WC>(m, l, source, k, rc, sh_off, sh_delta, prod);
 }
 // op 8 into level l (from l-1) with G lanes per coarse node over the fine
 // support.  With G > 1 the lanes' weighted children go to the node's LDS slice
@@ -1190,6 +1201,15 @@ static size_t coarse_lds(const MGDev& m, int l) {
   const MGLevel& L = m.lv[l];
   return L.lanes > 1 ? sizeof(float) * (256 / L.lanes) * L.width : 0;
 }
+// a k_mg_coarse sweep, with the compile-time width where the level allows it
+template <int OP, bool H>
+static void launch_coarse(const Ctx& s, const MGDev& m, int l, int source, int k, int phase, hipStream_t st) {
+  const MGLevel& L = m.lv[l];
+  const dim3 grid(nblocks((int64_t)L.count * L.lanes, 256));
+  // (a compile-time width of 39 with every load of a row issued up front was
+  // measured slower on level 1: 13.2 -> 15.0 us, occupancy 8 -> 4 waves)
+  hipLaunchKernelGGL((k_mg_coarse<OP, H, 0>), grid, dim3(256), coarse_lds(m, l), st, s, m, l, source, k, phase);
+}
 
 template <bool H>
 struct CycleBuilder {
@@ -1199,7 +1219,7 @@ struct CycleBuilder {
     for (int sweep = 0; sweep < smoothing; sweep++) {
       if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
       else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, sweep + 1, phase);
-      else hipLaunchKernelGGL((k_mg_coarse<0, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, field, sweep + 1, phase);
+      else launch_coarse<0, H>(s, m, l, field, sweep + 1, phase, st);
       field = 3 - field;
     }
     return field;
@@ -1216,7 +1236,7 @@ struct CycleBuilder {
       int tf = 1;
       for (int sweep = 0; sweep < 32; sweep++) {
         if (sweep == 0) hipLaunchKernelGGL(k_mg_jacobi, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, tf, 1, phase);
-        else hipLaunchKernelGGL((k_mg_coarse<2, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, tf, 0, phase);
+        else launch_coarse<2, H>(s, m, l, tf, 0, phase, st);
         tf = 3 - tf;
       }
       return tf;
@@ -1224,7 +1244,7 @@ struct CycleBuilder {
     int field = smooth(l, 1);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
     else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, 0, phase);
-    else hipLaunchKernelGGL((k_mg_coarse<1, H>), dim3(coop_blocks(l)), dim3(256), coarse_lds(m, l), st, s, m, l, field, 0, phase);
+    else launch_coarse<1, H>(s, m, l, field, 0, phase, st);
     hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), restrict_lds(m, l + 1), st, s, m, l + 1, phase);
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
@@ -1287,7 +1307,7 @@ int wv_time_copy(float* a, float* b, int64_t n, int reps, hipStream_t st, float*
 void wv_launch_timed(const Ctx& c, const MGDev& m, int kernel, hipStream_t st) {
   if (kernel == 4) {   // Chebyshev sweep 2 on level 1 (the largest coarse level), gates open
     const int l = m.L > 1 ? 1 : 0;
-    hipLaunchKernelGGL((k_mg_coarse<0, false>), dim3(nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256)), dim3(256), coarse_lds(m, l), st, c, m, l, 1, 2, 1);
+    launch_coarse<0, false>(c, m, l, 1, 2, 1, st);
     return;
   }
   if (kernel == 3) { hipLaunchKernelGGL(k_copy_timed, dim3(nblocks(c.N, 256)), dim3(256), 0, st, c.work + 3 * c.N, m.ap, c.N); return; }

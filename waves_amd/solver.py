@@ -33,7 +33,8 @@ EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_reques
            "wv_advance", "wv_poll_control", "wv_field_bytes", "wv_read_field",
            "wv_write_field", "wv_op", "wv_mg_record", "wv_synchronize", "wv_last_error",
            "wv_levels", "wv_plan_levels", "wv_time_kernel", "wv_pressure_solve", "wv_set_precision",
-           "wv_liquid_volume", "wv_get_schedule_stats", "wv_liquid_volume_noalloc")
+           "wv_liquid_volume", "wv_get_schedule_stats", "wv_liquid_volume_noalloc", "wv_save_state",
+           "wv_load_state")
 #: every entry point declared in include/waves_net.h
 NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
                "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
@@ -96,6 +97,8 @@ def library() -> ctypes.CDLL:
         L.wv_request_time.argtypes = [vp, ctypes.c_double]
         L.wv_advance.argtypes = [vp, i32, ctypes.c_float]
         L.wv_get_schedule_stats.argtypes = [vp, ctypes.POINTER(_ScheduleStats)]
+        L.wv_save_state.argtypes = [vp, ctypes.c_char_p]
+        L.wv_load_state.argtypes = [vp, ctypes.c_char_p]
         L.wv_liquid_volume_noalloc.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
         L.wv_poll_control.argtypes = [vp, fp, ctypes.POINTER(i32), i32]
         L.wv_field_bytes.restype = i64
@@ -318,6 +321,14 @@ class Solver:
     def write(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values)
         self._check(self._lib.wv_write_field(self._h, field, v.ctypes.data, v.nbytes))
+
+    def save_state(self, path: str):
+        """Write the whole solver state to a state file (wv_save_state)."""
+        self._check(self._lib.wv_save_state(self._h, os.fsencode(path)))
+
+    def load_state(self, path: str):
+        """Load a state file written for the same settings (wv_load_state)."""
+        self._check(self._lib.wv_load_state(self._h, os.fsencode(path)))
 
     def op(self, op: int):
         self._check(self._lib.wv_op(self._h, int(op)))
