@@ -223,19 +223,21 @@ __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int so
   coarse_row<OP, H, WC>(m, l, source, k, rc, sh_off, sh_delta, prod);
 }
 // op 8 into level l (from l-1) with G lanes per coarse node over the fine
-// support.  With G > 1 the lanes' weighted children go to the node's LDS slice
-// (child t at part[t * R + r], R nodes per block, with an active flag) and
-// lane 0 adds the active ones in child order, as the reference's loop does
-// (pressure_multigrid.glslinc:200-224), so the sum is bit-identical to one
-// thread walking the support.
-#define MG_RESTRICT_MAX 64   // children of a node: a 4x4x4 (trilinear) or 2x2x2 (aggregation) box
+// support.  The reference adds the active children in child order
+// (pressure_multigrid.glslinc:200-224); adding +0 for an inactive child
+// instead leaves every partial sum unchanged (the sum starts at +0, so it is
+// never -0), which lets every child contribute unconditionally.  With G > 1 the
+// lanes' weighted children go to the node's LDS slice (node-major, child t at
+// part[r * MG_RESTRICT_STRIDE + t]) and lane 0 adds them in child order, four
+// per LDS read, so the sum is bit-identical to one thread walking the support.
+#define MG_RESTRICT_MAX 64      // children of a node: a 4x4x4 (trilinear) or 2x2x2 (aggregation) box
+#define MG_RESTRICT_STRIDE 68   // slice stride: float4-aligned, successive nodes four banks apart
 __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
   if (!mg_enabled(s, phase)) return;   // uniform over the launch: no thread reaches a barrier
-  extern __shared__ float part[];      // (256 / G) nodes x 64 children when G > 1, then their flags
+  extern __shared__ float part[];      // (256 / G) node slices when G > 1
   const MGLevel& C = m.lv[l]; int parent = l - 1; const MGLevel& P = m.lv[parent];
   RowCtx rc = row_ctx(C.rlanes, C.count);
-  const int R = 256 / rc.G, r = (int)threadIdx.x / rc.G;
-  unsigned char* on_flag = reinterpret_cast<unsigned char*>(part + (rc.G > 1 ? R * MG_RESTRICT_MAX : 0));
+  float* mine = part + ((int)threadIdx.x / rc.G) * MG_RESTRICT_STRIDE;
   int total = 0;
   float sum = 0;
   I3 nf = mdims(m, parent), nl = mdims(m, l);
@@ -273,21 +275,26 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
           weight *= w;
         }
       }
-      if (rc.G == 1) { if (on) sum += weight * v; }
-      else { part[t * R + r] = weight * v; on_flag[t * R + r] = on ? 1 : 0; }
+      const float pv = on ? weight * v : 0.0f;
+      if (rc.G == 1) sum += pv;
+      else mine[t] = pv;
     }
   }
   if (rc.G > 1) {
     __syncthreads();
-    if (rc.valid && rc.lane == 0)
-      for (int t = 0; t < total; t++) if (on_flag[t * R + r]) sum += part[t * R + r];
+    if (rc.valid && rc.lane == 0) {
+      const float4* m4 = reinterpret_cast<const float4*>(mine);
+      int t = 0;
+      for (; t + 4 <= total; t += 4) { const float4 q = m4[t >> 2]; sum += q.x; sum += q.y; sum += q.z; sum += q.w; }
+      for (; t < total; t++) sum += mine[t];
+    }
   }
   if (!rc.valid || rc.lane != 0) return;
   m.vec[at(m, l, 0, rc.row)] = sum; m.vec[at(m, l, 1, rc.row)] = 0; m.vec[at(m, l, 2, rc.row)] = 0;
 }
 static size_t restrict_lds(const MGDev& m, int l) {
   const int G = m.lv[l].rlanes;
-  return G > 1 ? (sizeof(float) + 1) * (256 / G) * MG_RESTRICT_MAX : 0;
+  return G > 1 ? sizeof(float) * (256 / G) * MG_RESTRICT_STRIDE : 0;
 }
 
 // terminal smoother when coarsening stops above 64 nodes
