@@ -91,7 +91,8 @@ __device__ __forceinline__ float velocity_donor(const Ctx& s, I3 q, int axis, bo
 __device__ __forceinline__ float sample_tap(const Ctx& s, I3 lo, int xx, int y, int z, int axis, bool forward, bool direct,
                                             I3 fn, int acc, bool& supported) {
   if (direct) {
-    int64_t f = face_local(s, lo, axis) + (xx + (int64_t)fn.x * (y + (int64_t)fn.y * z));
+    // a direct corner reads the stored face at the wrapped z (sample_at)
+    int64_t f = face_local(s, mk(lo.x + xx, lo.y + y, pidx(lo.z + z, s.nz)), axis);
     supported = s.motion[valid_base(s, acc) + f] > .5f;
     return forward ? s.motion[forward_base(s) + f] : getv(s, acc, U_(s) + f);
   }
@@ -114,10 +115,14 @@ __device__ __forceinline__ SampleAt sample_at(const Ctx& s, F3 x, int axis) {
   a.t = mkf(fract(lattice.x), fract(lattice.y), fract(lattice.z));
   bool interior = (s.flags & 8) != 0 && lo.x >= 1 && lo.y >= 1 && lo.z >= 1 && lo.x + 1 < s.nx && lo.y + 1 < s.ny && lo.z + 1 < s.nz &&
                   !solid(s, mk(lo.x + 1, lo.y - 1, lo.z));
-  // All eight corners on velocity_donor's first branch (in range, no periodic
-  // x, both cells of every corner face not solid): the bed test is monotone
-  // in y, so the lowest cell of each touched column decides for the column.
-  bool fluid = s.perx < .5f && lo.x >= 0 && lo.y >= 0 && lo.z >= (axis == 2 ? 1 : 0) && lo.z + 1 < s.nz &&
+  // All eight corners read the stored face with parity 1 (in range in x and
+  // y, no periodic x, both cells of every corner face not solid): the bed test
+  // is monotone in y, so the lowest cell of each touched column decides for
+  // the column.  Neither the bed nor solid() depends on z (solid() wraps it),
+  // so a corner across the periodic z edge is the stored face at the wrapped
+  // z, which is what velocity_donor's general path reads there (wrap_face,
+  // then both cells non-solid).
+  bool fluid = s.perx < .5f && lo.x >= 0 && lo.y >= 0 &&
                lo.x + 1 < s.nx + (axis == 0 ? 1 : 0) && lo.y + 1 < s.ny + (axis == 1 ? 1 : 0);
   if (fluid) {
     int ylow = lo.y - (axis == 1 ? 1 : 0);
@@ -136,11 +141,14 @@ __device__ __forceinline__ void corner_update(F3 t, int c8, float v, bool sup, b
 __device__ __forceinline__ float sample_direct(const Ctx& s, const SampleAt& at, int axis, bool forward, bool& valid, float& lim_lo, float& lim_hi) {
   I3 fn = dims(s); addc(fn, axis, 1);
   int acc = accepted(s);
-  int64_t f0 = face_local(s, at.lo, axis), vb = valid_base(s, acc), fb = forward_base(s), ub = U_(s);
+  // the two z layers of the corners, wrapped over the periodic z edge
+  const int64_t fz0 = face_local(s, mk(at.lo.x, at.lo.y, pidx(at.lo.z, s.nz)), axis);
+  const int64_t fz1 = face_local(s, mk(at.lo.x, at.lo.y, pidx(at.lo.z + 1, s.nz)), axis);
+  int64_t vb = valid_base(s, acc), fb = forward_base(s), ub = U_(s);
   float v[8]; bool sup[8];
 #pragma unroll
   for (int c8 = 0; c8 < 8; c8++) {
-    int64_t f = f0 + ((c8 & 1) + (int64_t)fn.x * (((c8 >> 1) & 1) + (int64_t)fn.y * (c8 >> 2)));
+    int64_t f = (c8 >> 2 ? fz1 : fz0) + ((c8 & 1) + (int64_t)fn.x * ((c8 >> 1) & 1));
     sup[c8] = s.motion[vb + f] > .5f;
     v[c8] = forward ? s.motion[fb + f] : getv(s, acc, ub + f);
   }
