@@ -41,7 +41,7 @@ sys.path.insert(0, ROOT)
 
 from waves_amd.settings import benchmark_beach, default_beach  # noqa: E402
 from waves_amd import solver as S  # noqa: E402
-from waves_amd.solver import Solver, stopped  # noqa: E402
+from waves_amd.solver import FIELD_QUEUES, Solver, stopped  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "accepted sim-s per wall-s @310x48x80; pressure ms/step @1e-6 rel residual"
@@ -335,6 +335,8 @@ def main():
         rooflines.append({"bound": "hbm", "kernel": label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_all.get(key, {}).get("hbm_bytes_per_launch"),
                           "avg_us": us, "bytes_per_launch": nbytes, "frac_of_copy": achieved / copy_gbs})
+        if kid == 5:  # faces the forward trace left to the queued general-path pass
+            rooflines[-1]["queued_faces"] = int(solver.read(FIELD_QUEUES, np.uint32)[0])
     rooflines[0]["copy_GBs"] = copy_gbs
     rooflines[0]["copy"] = "streaming float4 copy, 1 GiB read + 1 GiB written"
     done_steps = max(steps, 1)

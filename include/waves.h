@@ -68,10 +68,12 @@ enum {
   WV_FIELD_MG_LINV = 17,      /* last level: dense L^-1, row-major */
   WV_FIELD_PRESSURE64 = 18,   /* N double: pressure of the last standalone solve in FP64 */
   WV_FIELD_PREVIOUS = 19,     /* 9 N float, the hierarchy audit's previous-trial operator copy */
-  WV_FIELD_STEP_TIMES = 20    /* capacity x 4 uint64 per accepted step k (row k - 1): device ns of the
+  WV_FIELD_STEP_TIMES = 20,   /* capacity x 4 uint64 per accepted step k (row k - 1): device ns of the
                                  accepted trial's pressure sections (hierarchy audit/rebuild, PCG, projection,
                                  defect corrections), device ns of every trial's pressure sections since
                                  the previous acceptance, the trial count c[32], the sections timed */
+  WV_FIELD_QUEUES = 21        /* 4 uint32, read only: faces or cells the last queued passes took on the
+                                 general path (MacCormack forward trace, MacCormack correction, exact distance) */
 };
 
 /* _init(settings) + start() + _initialize (water/pressure_gpu.gd:40-47, 82-127):

@@ -49,6 +49,7 @@ struct Ctx {
   int flex;            // flexible (Polak-Ribiere) CG coefficient in the FP64 V-cycle output
   int* mq;             // faces whose MacCormack trace needs the general sampling path (NF)
   unsigned* mq_count;  // queue lengths: [0] forward trace (op 27), [1] correction (op 28)
+  signed char* reflect;  // mirror code of the wall search per (component, y, x), 3 (ny + 1) (nx + 1)
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
   // pressure-section timing on the device clock (telemetry, no reference counterpart)
   unsigned long long* tstate;      // [0] section start, [1] section active, [2] trial ns, [3] ns since last acceptance,

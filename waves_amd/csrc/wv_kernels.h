@@ -45,6 +45,7 @@ struct MGDev {
 int wv_launch_op(const Ctx& s, int op, hipStream_t st);
 // pressure-section timing: begin / end stamps around a batch's pressure
 // section, and the per-accepted-step record after the acceptance op
+void wv_launch_reflect_table(const Ctx& s, hipStream_t st);  // once per solver: the static wall-search table
 void wv_launch_press_begin(const Ctx& s, hipStream_t st);
 void wv_launch_press_end(const Ctx& s, hipStream_t st);
 void wv_launch_step_record(const Ctx& s, hipStream_t st);

@@ -24,9 +24,15 @@ __device__ __forceinline__ float old_face(const Ctx& s, I3 q, int axis) {
 }
 __device__ __forceinline__ float face_courant(const Ctx& s, I3 q, int axis) { return old_face(s, q, axis) * timestep(s) / s.h[axis]; }
 
-// coupled_motion.glslinc:66-85: nearest wall within three layers
-__device__ __forceinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int component, float& parity) {
-  float best = 1e20f; I3 selected = q; parity = 1;
+// coupled_motion.glslinc:66-85: nearest wall within three layers.  The search
+// returns the chosen mirror as an offset code, (axis << 4) | (8 if the step is
+// positive) | k with the mirror k faces along axis, or -1 when no wall is in
+// reach.  It reads solid() only, and solid() does not depend on z (the bed is
+// a function of x and y, and z wraps), so the code for a face depends on its
+// component, x and y alone: Ctx::reflect holds it for every in-range face at
+// z = 0, filled once when the solver is created (k_reflect_table).
+__device__ __forceinline__ int reflect_search(const Ctx& s, I3 q, int component) {
+  float best = 1e20f; int code = -1;
   for (int axis = 0; axis < 3; axis++) for (int sign = -1; sign <= 1; sign += 2) {
     for (int layer = 1; layer <= 3; layer++) {
       I3 edge = q; addc(edge, axis, sign * layer); edge = wrap_face(s, edge);
@@ -35,15 +41,26 @@ __device__ __forceinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int com
       bool sl = solid(s, left), sr = solid(s, edge);
       bool boundary = axis == component ? (sl != sr) : (!sl && !sr);
       if (!boundary) continue;
-      I3 mirror = q; addc(mirror, axis, sign * (2 * layer - (axis == component ? 0 : 1))); mirror = wrap_face(s, mirror);
+      const int k = 2 * layer - (axis == component ? 0 : 1);
+      I3 mirror = q; addc(mirror, axis, sign * k); mirror = wrap_face(s, mirror);
       float distance = ((float)layer - (axis == component ? 0 : .5f)) * s.h[axis];
       if (face_in_bounds(s, mirror, component) && !wall_face(s, mirror, component) && distance < best) {
-        selected = mirror; best = distance; parity = axis == component ? -1 : 1;
+        code = (axis << 4) | (sign > 0 ? 8 : 0) | k; best = distance;
       }
       break;
     }
   }
-  q = selected; return best < 1e19f;
+  return code;
+}
+// the mirror face and parity of a face between two solid cells (q wrapped, in range)
+__device__ __forceinline__ bool reflect_voxel_donor(const Ctx& s, I3& q, int component, float& parity) {
+  const int code = s.reflect[((int64_t)component * (s.ny + 1) + q.y) * (s.nx + 1) + q.x];
+  parity = 1;
+  if (code < 0) return false;
+  const int axis = code >> 4;
+  addc(q, axis, (code & 8) ? (code & 7) : -(code & 7)); q = wrap_face(s, q);
+  parity = axis == component ? -1 : 1;
+  return true;
 }
 // solid() for a cell whose z is already in [0, nz) without periodic x, where
 // the periodic wrap is the identity

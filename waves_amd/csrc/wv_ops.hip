@@ -497,6 +497,21 @@ __global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s) {
   for (int64_t j = GID() / 8; j < n; j += groups) maccormack_face<CORRECT, true, false, true>(s, s.mq[j], sub);
 }
 
+// The wall-search code of every in-range face at z = 0, per (component, y, x)
+// (the search does not depend on z); -1 where no face is in range.
+__global__ void __launch_bounds__(256) k_reflect_table(Ctx s) {
+  const int64_t i = GID(), row = (int64_t)(s.nx + 1) * (s.ny + 1);
+  if (i >= 3 * row) return;
+  const int component = (int)(i / row), r = (int)(i - component * row);
+  const int y = r / (s.nx + 1), x = r - y * (s.nx + 1);
+  const I3 q = mk(x, y, 0);
+  s.reflect[i] = (signed char)(face_in_bounds(s, q, component) ? reflect_search(s, q, component) : -1);
+}
+void wv_launch_reflect_table(const Ctx& s, hipStream_t st) {
+  const int64_t n = 3 * (int64_t)(s.nx + 1) * (s.ny + 1);
+  hipLaunchKernelGGL(k_reflect_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s);
+}
+
 // --------------------------------------------------- coastal relaxation -----
 __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
   F3 v;

@@ -288,6 +288,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.aux, 2 * c.N);
   ALLOC(c.mq, c.NF);
   ALLOC(c.mq_count, 4);
+  ALLOC(c.reflect, 3 * (int64_t)(c.nx + 1) * (c.ny + 1));
   ALLOC(c.tstate, 8);
   ALLOC(c.step_times, 4 * (s->history_capacity + 1));
   ALLOC(s->ctl_timing, WV_NCTRL);
@@ -330,6 +331,8 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
     c.tick_ns = (float)(1e6 / khz);
   }
+  wv_launch_reflect_table(c, s->stream);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) { release(s); delete s; return WV_EHIP; }
   *out = s;
   return WV_OK;
 }
@@ -585,6 +588,7 @@ static void* field_ptr(wv_solver* s, int field, int64_t* bytes) {
   case WV_FIELD_PRESSURE64: *bytes = c.N * 8; return c.pacc;
   case WV_FIELD_PREVIOUS: *bytes = 9 * c.N * 4; return m.previous;
   case WV_FIELD_STEP_TIMES: *bytes = s->history_capacity * 4 * 8; return c.step_times;
+  case WV_FIELD_QUEUES: *bytes = 4 * 4; return c.mq_count;
   }
   *bytes = -1;
   return nullptr;
@@ -607,7 +611,7 @@ extern "C" int wv_write_field(wv_solver* s, int field, const void* src, int64_t 
   if (!s || !src) return WV_EINVAL;
   std::lock_guard<std::mutex> g(s->lock);
   int64_t n; void* p = field_ptr(s, field, &n);
-  if (!p || bytes < 0 || bytes > n) return fail(s, WV_EINVAL, "bad field or size");
+  if (!p || bytes < 0 || bytes > n || field == WV_FIELD_QUEUES) return fail(s, WV_EINVAL, "bad field or size");
   HIPCHECK(s, hipStreamSynchronize(s->stream));
   HIPCHECK(s, hipMemcpy(p, src, (size_t)bytes, hipMemcpyHostToDevice));
   return WV_OK;

@@ -25,7 +25,7 @@ FIELD_BANK_A, FIELD_BANK_B, FIELD_WORK, FIELD_GEO, FIELD_CONTROL = 0, 1, 2, 3, 4
 FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
 FIELD_HISTORY, FIELD_COASTAL_HISTORY, FIELD_PROBE, FIELD_PARTIAL = 9, 10, 11, 12
 FIELD_MG_VECTORS, FIELD_SNAPSHOT, FIELD_AUDIT, FIELD_MG_ROWS, FIELD_MG_LINV = 13, 14, 15, 16, 17
-FIELD_PRESSURE64, FIELD_PREVIOUS, FIELD_STEP_TIMES = 18, 19, 20
+FIELD_PRESSURE64, FIELD_PREVIOUS, FIELD_STEP_TIMES, FIELD_QUEUES = 18, 19, 20, 21
 MAX_INFLIGHT = 3
 
 #: every entry point declared in include/waves.h
