@@ -151,7 +151,10 @@ int wv_save_state(wv_solver *s, const char *path);
 int wv_load_state(wv_solver *s, const char *path);
 /* one reference op (water/pressure_gpu.gd:69-81 _dispatch without the MG hook),
  * for the parity harness; ops whose finaliser is fused into the previous op's
- * kernel are no-ops. Enqueued, not waited for. */
+ * kernel are no-ops. Enqueued, not waited for.  WV_OP_SPMV_DOT (no reference
+ * counterpart) runs ops 5, 6 and 7 in the one-pass form the plain-FP32 batch
+ * and standalone solve use. */
+#define WV_OP_SPMV_DOT 105
 int wv_op(wv_solver *s, int op);
 /* multigrid.record(list, build, initial) (water/pressure_multigrid.gd:155-176) */
 int wv_mg_record(wv_solver *s, int build, int initial);

@@ -29,16 +29,25 @@ def batch_groups(iterations: int = 8):
         g += [(23,), (24,), (55, 56), (25, 26)]
     g += [(27,), (28,), (80, 81), (82,), (83, 84), (87, 88), (17,), (18,), (16,), (2,), (47,), (63,), (64,),
           ("build",), (3, "cycle0", 4)]
-    for _ in range(iterations):
-        g += [(5,), (6, 7), (8, "cycle1", 9), (10,)]
+    for it in range(iterations):
+        # the batch runs ops 5-7 in one pass (SPMV_DOT); the first iteration
+        # also checks the three ops one by one
+        g += [(5,), (6, 7)] if it == 0 else [(SPMV_DOT,)]
+        g += [(8, "cycle1", 9), (10,)]
     g += [(11,), (65,), (47,), (30,), (31,), (32,), (33,), (34,), (47,), (48,), (35, 36), (62, 66), (12, 13),
           (44,), (19,), (85,), (89,)]
     return g
 
 
+SPMV_DOT = "spmv_dot"  # ops 5, 6, 7 fused (WV_OP_SPMV_DOT on the GPU)
+
+
 def run_oracle(o: O.Oracle, group):
     for op in group:
-        if op == "build":
+        if op == SPMV_DOT:
+            for k in (5, 6, 7):
+                o.op(k)
+        elif op == "build":
             o.mg_record(True)
         elif op == "cycle0":
             o.mg_record(False, True)
@@ -50,7 +59,9 @@ def run_oracle(o: O.Oracle, group):
 
 def run_gpu(g: S.Solver, group):
     for op in group:
-        if op == "build":
+        if op == SPMV_DOT:
+            g.op(S.OP_SPMV_DOT)
+        elif op == "build":
             g.mg_record(True)
         elif op in ("cycle0", "cycle1"):
             g.mg_record(False, op == "cycle0")

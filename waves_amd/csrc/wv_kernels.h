@@ -50,6 +50,7 @@ void wv_launch_press_begin(const Ctx& s, hipStream_t st);
 void wv_launch_press_end(const Ctx& s, hipStream_t st);
 void wv_launch_step_record(const Ctx& s, hipStream_t st);
 void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st);
+void wv_launch_spmv_dot(const Ctx& s, hipStream_t st);  // ops 5 + 6 + 7 (plain FP32 form)
 // Multigrid: build = hierarchy audit + Galerkin rebuild (phase 2);
 // cycle = one V-cycle (phase 0 initial / 1 iteration) ending in the fused
 // z write-back, partials and PCG finaliser (op 4 or op 9).

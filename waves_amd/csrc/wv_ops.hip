@@ -724,6 +724,21 @@ __global__ void __launch_bounds__(256) k_spmv(Ctx s) {
   int64_t i = GID(); if (i >= s.N) return;
   I3 q = coord(i, dims(s)); s.work[3 * s.N + i] = wet(s, q) ? operator_at(s, q, 1) : 0;
 }
+// ops 5 + 6 in one pass: A p of every cell and, per 128-cell group, the p.Ap
+// partial of op 6 from the value just computed (the same groups and the same
+// combine tree as k_dot_alpha's first pass, so the partials are identical)
+__global__ void __launch_bounds__(WV_RED) k_spmv_dot(Ctx s) {
+  if (!solving(s)) return;
+  const int64_t i = GID(), N = s.N;
+  float prod = 0;
+  if (i < N) {
+    const I3 q = coord(i, dims(s));
+    const float ap = wet(s, q) ? operator_at(s, q, 1) : 0;
+    s.work[3 * N + i] = ap; prod = s.work[2 * N + i] * ap;
+  }
+  const float4 v = group_reduce<RED_SUM>(make_float4(prod, 0, 0, 0));
+  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+}
 // pressure_iterations.glslinc:21-22 (ops 6 + 7)
 __global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, int final) {
   if (!solving(s)) return;
@@ -1060,6 +1075,11 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   default: return -1;
   }
   return 0;
+}
+// ops 5, 6 and 7 of one PCG iteration: A p with the p.Ap partials, then alpha
+void wv_launch_spmv_dot(const Ctx& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_spmv_dot, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s);
+  hipLaunchKernelGGL(k_dot_alpha, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
 void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st) {
   hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, b0, x1, x2);

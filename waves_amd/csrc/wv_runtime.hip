@@ -360,6 +360,7 @@ static int sync_quota(wv_solver* s) {
 // water/pressure_gpu.gd:69-81: one op; the MG hooks are explicit in the batch.
 static int dispatch(wv_solver* s, int op) {
   const Ctx& c = s->ctx;
+  if (op == WV_OP_SPMV_DOT) { wv_launch_spmv_dot(c, s->stream); return 0; }
   if (op == 3) {
     wv_launch_op(c, 3, s->stream);
     wv_launch_pcg_init_partials(c, s->stream);
@@ -395,9 +396,8 @@ static int record_batch(wv_solver* s) {
   if (mixed) wv_mg_cycle_bf16(s->ctx, s->mg, 0, s->smoothing, true, true, s->stream);  // + op 4 in FP64
   else wv_mg_cycle(s->ctx, s->mg, 0, s->smoothing, true, s->stream);                  // + op 4
   for (int it = 0; it < s->iterations; it++) {
-    d(5);
-    if (mixed) wv_solve_dot_alpha64(s->ctx, s->stream);   // ops 6 + 7 in FP64
-    else { d(6); d(7); }
+    if (mixed) { d(5); wv_solve_dot_alpha64(s->ctx, s->stream); }   // ops 6 + 7 in FP64
+    else wv_launch_spmv_dot(s->ctx, s->stream);                      // ops 5 + 6 + 7
     d(8);
     if (mixed) wv_mg_cycle_bf16(s->ctx, s->mg, 1, s->smoothing, false, true, s->stream);  // + op 9 in FP64
     else wv_mg_cycle(s->ctx, s->mg, 1, s->smoothing, false, s->stream);                  // + op 9
@@ -672,10 +672,13 @@ uint64_t state_sum(const unsigned char* p, size_t n) {
   return h;
 }
 int field_elem(int id) { return id == WV_FIELD_STEP_TIMES ? 8 : 4; }
-// settings that define the buffers and the physics; the op and quota slots are per dispatch
+// settings that define the buffers and the physics; the op and quota slots are
+// per dispatch, and the PCG iteration slots per batch (opts[5]) only split the
+// same gated program differently
 bool same_settings(const wv_settings& a, const float* pc, const float* opts) {
   for (int k = 0; k < 32; k++) if (k != 3 && k != 15 && memcmp(&a.pc[k], &pc[k], 4) != 0) return false;
-  return memcmp(a.opts, opts, sizeof a.opts) == 0;
+  for (int k = 0; k < 8; k++) if (k != 5 && memcmp(&a.opts[k], &opts[k], 4) != 0) return false;
+  return true;
 }
 }  // namespace
 
@@ -865,8 +868,8 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
   // measured 7 % slower than these direct launches.)
   auto iterate = [&](int n) {
     for (int k = 0; k < n; k++) {
-      wv_launch_op(c, 5, st);                  // A p
-      if (mode >= 1) wv_solve_dot_alpha64(c, st); else wv_launch_op(c, 6, st);
+      if (mode >= 1) { wv_launch_op(c, 5, st); wv_solve_dot_alpha64(c, st); }   // A p, alpha in FP64
+      else wv_launch_spmv_dot(c, st);          // A p with the p.Ap partials, alpha
       wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, st);
       cycle(1, false);                         // z = M r, beta, convergence (op 9)
       wv_launch_op(c, 10, st);                 // p = z + beta p

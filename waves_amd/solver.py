@@ -26,6 +26,7 @@ FIELD_PLANE, FIELD_MOTION, FIELD_IMG_PLANES, FIELD_IMG_KINDS = 5, 6, 7, 8
 FIELD_HISTORY, FIELD_COASTAL_HISTORY, FIELD_PROBE, FIELD_PARTIAL = 9, 10, 11, 12
 FIELD_MG_VECTORS, FIELD_SNAPSHOT, FIELD_AUDIT, FIELD_MG_ROWS, FIELD_MG_LINV = 13, 14, 15, 16, 17
 FIELD_PRESSURE64, FIELD_PREVIOUS, FIELD_STEP_TIMES, FIELD_QUEUES = 18, 19, 20, 21
+OP_SPMV_DOT = 105  # ops 5, 6, 7 in the batch's one-pass form (no reference counterpart)
 MAX_INFLIGHT = 3
 
 #: every entry point declared in include/waves.h
