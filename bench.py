@@ -44,6 +44,7 @@ from waves_amd import solver as S  # noqa: E402
 from waves_amd.solver import FIELD_QUEUES, Solver, stopped  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16
 METRIC = "accepted sim-s per wall-s @310x48x80; pressure ms/step @1e-6 rel residual"
 WINDOW_START = 8.0      # sim-s, breaking phase
 FULL_RUN = 20.0         # sim-s
@@ -179,10 +180,41 @@ def pressure_leg(repeats: int = 3, n=(310, 48, 80), spacing=(0.025, 0.025, 0.05)
         out[f"speedup_mixed@{tol:g}"] = out[f"plain_fp32@{tol:g}"]["ms"] / out[f"mixed_bf16_fp64@{tol:g}"]["ms"]
         if net is not None:
             out[f"speedup_learned@{tol:g}"] = out[f"plain_fp32@{tol:g}"]["ms"] / out[f"mixed_learned@{tol:g}"]["ms"]
+    if net is not None:
+        out["net_layers"] = net_layer_roofline(net)
     g.close()
     if net is not None:
         net.close()
     return out
+
+
+def net_layer_roofline(net, reps: int = 20):
+    """Per-layer roofline of the warm-start network's inference (batch 1, on
+    the features of the last learned solve): each 3x3x3 convolution is an
+    implicit GEMM of 2 x cells x 27 cin x cout flops (bias excluded); its
+    algorithmic bytes are the input read once (ca channels at the layer's
+    level, cb at the next coarser level, bf16), the output written once (bf16
+    after a ReLU, fp32 for the two heads) and the bf16 weights.  Times are HIP
+    events around `reps` launches of the layer on the network's stream."""
+    us = net.time_layers(reps)
+    rows, tot_f, tot_us = [], 0.0, 0.0
+    names = ("e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b3a", "b3b", "d2a", "d2b", "hc", "d1a", "d1b", "d0a", "d0b", "hf")
+    for name, L, t in zip(names, net.layers(), us):
+        cin = L["ca"] + L["cb"]
+        flops = 2.0 * L["cells"] * 27 * cin * L["cout"]
+        kp = (27 * cin + 1 + 15) // 16 * 16
+        nbytes = (2.0 * (L["cells"] * L["ca"] + L["coarse_cells"] * L["cb"]) + L["cells"] * L["cout"] * (4 if L["cout"] == 1 else 2)
+                  + 2.0 * L["cout"] * kp)
+        tf = flops / (t * 1e-6) / 1e12
+        gbs = nbytes / (t * 1e-6) / 1e9
+        rows.append({"layer": name, "level": L["level"], "cells": L["cells"], "cin": cin, "cout": L["cout"], "us": float(t),
+                     "tflops": tf, "frac_mfma_bf16": tf / MFMA_BF16_PEAK_TFLOPS, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
+                     "intensity_flop_per_byte": flops / nbytes,
+                     "bound": "mfma" if flops / nbytes > MFMA_BF16_PEAK_TFLOPS * 1e3 / HBM_PEAK_GBS else "hbm"})
+        tot_f += flops
+        tot_us += float(t)
+    return {"layers": rows, "sum_us": tot_us, "gflop": tot_f / 1e9, "tflops": tot_f / (tot_us * 1e-6) / 1e12,
+            "mfma": "v_mfma_f32_16x16x32_bf16 (K step 32) + one 16x16x16 step where kpad is 16 short of a multiple of 32"}
 
 
 def inloop_leg(steps: int = 600, tol: float = 1e-6):

@@ -44,6 +44,17 @@ int wv_net_train(wv_net *net, int steps, int train_count, float lr, uint64_t see
 /* Mean loss over dataset samples [first, first + count) (batches of `batch`). */
 int wv_net_eval(wv_net *net, int first, int count, float *loss);
 const char *wv_net_last_error(wv_net *net);
+/* The network's layers in forward order: count, and layer l's grid level
+ * (0 = full grid, each level halves every axis rounding up), input channels
+ * at that level (ca), input channels upsampled from the next level (cb) and
+ * output channels. */
+int wv_net_layer_count(void);
+int wv_net_layer_info(int l, int *level, int *ca, int *cb, int *cout);
+/* One forward pass of the batch already in the network's input buffer (the
+ * last wv_net_forward's), then every layer's convolution launched `reps` times
+ * on the network's stream and timed with HIP events: us[l] = mean device
+ * microseconds per launch of layer l (count = wv_net_layer_count()). */
+int wv_net_time_layers(wv_net *net, int reps, float *us, int count);
 
 /* Attach a network (created with the solver's grid and batch 1) for the
  * learned warm start of wv_pressure_solve mode 2; NULL detaches.  The solver

@@ -1,5 +1,7 @@
 // Learned warm-start network: 3-D conv U-Net training and inference on
-// CDNA4 bf16 MFMA (v_mfma_f32_16x16x16_bf16), fp32 accumulation.
+// CDNA4 bf16 MFMA, fp32 accumulation: the convolutions step K by 32 with
+// v_mfma_f32_16x16x32_bf16 (gfx950) and finish a K that is 16 short of a
+// multiple of 32 with one v_mfma_f32_16x16x16_bf16.
 //
 // Layout: activations channels-last, cell index x + nx (y + ny (z + nz b)),
 // bf16; activation gradients fp32.  A 3x3x3 convolution is an implicit GEMM
@@ -21,7 +23,14 @@
 
 typedef uint16_t bf16_t;
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+// 16x16 output tile += A (16 x 32) B (32 x 16): lane l holds A[l % 16][8 (l / 16) + 0..7]
+// and B[8 (l / 16) + 0..7][l % 16]; the accumulator layout is the 16x16x16 one
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
 
 struct Grid { int nx, ny, nz, B; };
 
@@ -72,20 +81,38 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   if (live) decode(c, a.g, x, y, z, b);
   v4f acc[NT];
   for (int t = 0; t < NT; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < a.kp; k0 += 16) {
-    const int k = k0 + 4 * kq;
-    v4s av = {0, 0, 0, 0};
+  // the A fragment of KW consecutive k from k (within one tap: cin, ca and cb
+  // are multiples of 8), the bias column's 1 at k == kreal
+  auto a_frag = [&](int k, auto zero) {
+    using V = decltype(zero);
+    V av = zero;
     if (live && k < kreal) {
       const int tap = k / cin, ch = k - tap * cin;
       const int qx = x + tap % 3 - 1, qy = y + (tap / 3) % 3 - 1, qz = z + tap / 9 - 1;
       if (inside(a.g, qx, qy, qz)) {
         const bf16_t* src = ch < a.ca ? a.xa + encode(a.g, qx, qy, qz, b) * a.ca + ch
                                       : a.xb + encode(a.gb, qx >> 1, qy >> 1, qz >> 1, b) * a.cb + (ch - a.ca);
-        av = *(const v4s*)src;
+        av = *(const V*)src;
       }
     } else if (live && k == kreal && a.bias) {
-      av = v4s{(short)0x3f80, 0, 0, 0};
+      av[0] = (short)0x3f80;
     }
+    return av;
+  };
+  int k0 = 0;
+  for (; k0 + 32 <= a.kp; k0 += 32) {
+    const int k = k0 + 8 * kq;
+    const v8s av = a_frag(k, v8s{0, 0, 0, 0, 0, 0, 0, 0});
+    for (int t = 0; t < NT; t++) {
+      const int co = t * 16 + m;
+      v8s bv = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (co < a.cout) bv = *(const v8s*)(a.w + (int64_t)co * a.kp + k);
+      acc[t] = mfma32(av, bv, acc[t]);
+    }
+  }
+  if (k0 < a.kp) {                             // kp = 32 j + 16
+    const int k = k0 + 4 * kq;
+    const v4s av = a_frag(k, v4s{0, 0, 0, 0});
     for (int t = 0; t < NT; t++) {
       const int co = t * 16 + m;
       v4s bv = {0, 0, 0, 0};
@@ -172,8 +199,32 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
   constexpr int KREAL = 27 * CIN;
   v4f acc[4][NT];
   for (int r = 0; r < 4; r++) for (int t = 0; t < NT; t++) acc[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  static_assert(CIN % 8 == 0, "an 8-wide K fragment stays within one tap");
+  // halo offset of this lane's fragment at k, or -1 past the taps
+  auto tap_offset = [&](int k) {
+    if (k >= KREAL) return -1;
+    const int tap = k / CIN, ch = k - tap * CIN;
+    return ((tap / 9) * (HX * HY) + ((tap / 3) % 3) * HX + (tap % 3) + m) * CIN + ch;
+  };
+  int k0 = 0;
 #pragma unroll 2
-  for (int k0 = 0; k0 < a.kp; k0 += 16) {
+  for (; k0 + 32 <= a.kp; k0 += 32) {
+    const int k = k0 + 8 * kq;
+    v8s bv[NT];
+    for (int t = 0; t < NT; t++) {
+      const int co = t * 16 + m;
+      if (WLDS) bv[t] = *(const v8s*)(wl + co * a.kp + k);
+      else bv[t] = co < a.cout ? *(const v8s*)(a.w + (int64_t)co * a.kp + k) : v8s{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    const int off = tap_offset(k);
+    for (int r = 0; r < 4; r++) {                       // row r of this wave: y = wave, z = r
+      v8s av = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (off >= 0) av = *(const v8s*)(halo + off + (r * (HX * HY) + wave * HX) * CIN);
+      else if (k == KREAL) av[0] = (short)0x3f80;
+      for (int t = 0; t < NT; t++) acc[r][t] = mfma32(av, bv[t], acc[r][t]);
+    }
+  }
+  if (k0 < a.kp) {                                     // kp = 32 j + 16
     const int k = k0 + 4 * kq;
     v4s bv[NT];
     for (int t = 0; t < NT; t++) {
@@ -181,12 +232,8 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
       if (WLDS) bv[t] = *(const v4s*)(wl + co * a.kp + k);
       else bv[t] = co < a.cout ? *(const v4s*)(a.w + (int64_t)co * a.kp + k) : v4s{0, 0, 0, 0};
     }
-    int off = -1;
-    if (k < KREAL) {
-      const int tap = k / CIN, ch = k - tap * CIN;
-      off = ((tap / 9) * (HX * HY) + ((tap / 3) % 3) * HX + (tap % 3) + m) * CIN + ch;
-    }
-    for (int r = 0; r < 4; r++) {                       // row r of this wave: y = wave, z = r
+    const int off = tap_offset(k);
+    for (int r = 0; r < 4; r++) {
       v4s av = {0, 0, 0, 0};
       if (off >= 0) av = *(const v4s*)(halo + off + (r * (HX * HY) + wave * HX) * CIN);
       else if (k == KREAL) av = v4s{(short)0x3f80, 0, 0, 0};
@@ -264,6 +311,9 @@ static bool launch_tiled(const ConvArgs& a, hipStream_t st) {
   }
 }
 
+// every input width (and padded output width, the input gradient's input) is
+// a multiple of 8, checked by wv_net_create: a 32-wide K fragment stays within
+// one tap and one input tensor
 static void launch_conv(const ConvArgs& a, hipStream_t st) {
   if (launch_tiled(a, st)) return;
   const unsigned g = nblk(cells_of(a.g), 64);
@@ -580,6 +630,8 @@ extern "C" void wv_net_destroy(wv_net* n) {
 extern "C" int wv_net_create(int nx, int ny, int nz, int batch, wv_net** out) {
   if (!out || nx < 1 || ny < 1 || nz < 1 || batch < 1) return WV_EINVAL;
   *out = nullptr;
+  for (int l = 0; l < NL; l++)
+    if (DEFS[l].ca % 8 != 0 || DEFS[l].cb % 8 != 0 || coutp_of(DEFS[l].cout) % 8 != 0) return WV_EINVAL;
   wv_net* n = new wv_net();
   n->batch = batch;
   n->lv[0] = Grid{nx, ny, nz, batch};
@@ -654,6 +706,8 @@ static const bf16_t* layer_input(wv_net* n, int l) {
   return d.src < 0 ? n->x : n->act[d.src];
 }
 
+// the forward convolution of layer l (its inputs computed)
+static void forward_conv(wv_net* n, int l, hipStream_t st);
 static void forward(wv_net* n, hipStream_t st) {
   for (int l = 0; l < NL; l++) {
     const LayerDef& d = DEFS[l];
@@ -661,17 +715,21 @@ static void forward(wv_net* n, hipStream_t st) {
       const Grid f = n->lv[d.level - 1], c = n->lv[d.level];
       hipLaunchKernelGGL(k_pool, dim3(nblk(cells_of(c) * d.ca, 256)), dim3(256), 0, st, n->act[d.src], f, c, d.ca, n->pooled[l]);
     }
-    ConvArgs a{};
-    a.xa = layer_input(n, l);
-    a.xb = d.kind == SRC_CAT ? n->act[d.src2] : nullptr;
-    a.w = n->wb + n->woff[l];
-    a.g = n->lv[d.level];
-    a.gb = d.kind == SRC_CAT ? n->lv[d.level + 1] : a.g;
-    a.ca = d.ca; a.cb = d.cb; a.kp = kpad(d.ca + d.cb); a.cout = d.cout; a.bias = 1; a.relu = d.relu;
-    if (d.relu) { a.yb = n->act[l]; a.ystride = d.cout; }
-    else { a.yf = l == HF ? n->qf : n->qc; a.ystride = 1; }
-    launch_conv(a, st);
+    forward_conv(n, l, st);
   }
+}
+static void forward_conv(wv_net* n, int l, hipStream_t st) {
+  const LayerDef& d = DEFS[l];
+  ConvArgs a{};
+  a.xa = layer_input(n, l);
+  a.xb = d.kind == SRC_CAT ? n->act[d.src2] : nullptr;
+  a.w = n->wb + n->woff[l];
+  a.g = n->lv[d.level];
+  a.gb = d.kind == SRC_CAT ? n->lv[d.level + 1] : a.g;
+  a.ca = d.ca; a.cb = d.cb; a.kp = kpad(d.ca + d.cb); a.cout = d.cout; a.bias = 1; a.relu = d.relu;
+  if (d.relu) { a.yb = n->act[l]; a.ystride = d.cout; }
+  else { a.yf = l == HF ? n->qf : n->qc; a.ystride = 1; }
+  launch_conv(a, st);
 }
 
 // loss of the current batch (x, t) into *dloss_slot, gradients into n->g
@@ -727,6 +785,34 @@ static void loss_backward(wv_net* n, float* dloss_slot, bool backward, hipStream
                          n->dact[d.src2], d.cb);
     }
   }
+}
+
+extern "C" int wv_net_layer_count(void) { return NL; }
+extern "C" int wv_net_layer_info(int l, int* level, int* ca, int* cb, int* cout) {
+  if (l < 0 || l >= NL || !level || !ca || !cb || !cout) return WV_EINVAL;
+  *level = DEFS[l].level; *ca = DEFS[l].ca; *cb = DEFS[l].cb; *cout = DEFS[l].cout;
+  return WV_OK;
+}
+extern "C" int wv_net_time_layers(wv_net* n, int reps, float* us, int count) {
+  if (!n || !us || count < NL || reps < 1) return WV_EINVAL;
+  forward(n, n->stream);
+  hipEvent_t e0, e1;
+  NCHECK(n, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return net_fail(n, "hipEventCreate", hipErrorUnknown); }
+  int rc = WV_OK;
+  for (int l = 0; l < NL && rc == WV_OK; l++) {
+    forward_conv(n, l, n->stream);                          // warm
+    hipError_t e = hipEventRecord(e0, n->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) forward_conv(n, l, n->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, n->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = net_fail(n, "layer timing", e);
+    else us[l] = ms * 1000.0f / (float)reps;
+  }
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return rc;
 }
 
 extern "C" int wv_net_forward(wv_net* n, const uint16_t* x, float* qf, float* qc) {

@@ -143,14 +143,22 @@ def run_settings(amplitude: float, period: float, n=None, tol: float = 1e-6) -> 
     return base
 
 
-def build_dataset(runs: int = 8, steps: int = 1000, every: int = 4, skip: int = 0, seed: int = 2024, log=print):
-    """(x bf16 bits (P, cells*8), t (P, cells), meta) from `runs` narrow-flume
-    runs with seeded amplitudes in [0.04, 0.10] m and periods in [1.2, 1.8] s."""
+def beach_settings(amplitude: float, period: float, tol: float = 1e-4) -> Settings:
+    """The 310x48x80 beach of the benchmark (seeded 1e-3 m perturbation) with
+    another incident wave; PCG tolerance as in the benchmark's loop."""
+    return default_beach(amplitude=amplitude, period=period, pressure_tolerance=tol, perturbation=1e-3, seed=1)
+
+
+def build_dataset(runs: int = 8, steps: int = 1000, every: int = 4, skip: int = 0, seed: int = 2024, log=print,
+                  scenario: str = "flume"):
+    """(x bf16 bits (P, cells*8), t (P, cells), meta) from `runs` runs with
+    seeded amplitudes in [0.04, 0.10] m and periods in [1.2, 1.8] s, on the
+    narrow flume or (scenario "beach") the 310x48x80 beach."""
     rng = np.random.default_rng(seed)
     xs, ts, meta = [], [], []
     for r in range(runs):
         amp, per = float(rng.uniform(0.04, 0.10)), float(rng.uniform(1.2, 1.8))
-        st = run_settings(amp, per)
+        st = beach_settings(amp, per) if scenario == "beach" else run_settings(amp, per)
         t0 = time.time()
         recs = capture(st, steps, every, skip)
         for rec in recs:
@@ -180,14 +188,16 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def fit(x: np.ndarray, t: np.ndarray, n, batch: int = 8, steps: int = 4000, lr: float = 1e-3,
-        val_fraction: float = 0.1, seed: int = 7, init_seed: int = 0, log=print):
+        val_fraction: float = 0.1, seed: int = 7, init_seed: int = 0, log=print, init=None):
     """Adam on the device; the last `val_fraction` of the pairs is held out.
-    Learning rate: `lr` for the first 70 % of the steps, then lr / 10."""
+    Learning rate: `lr` for the first 70 % of the steps, then lr / 10.  The
+    weights start from `init` (a parameter vector, to fine-tune) or from
+    He-normal draws."""
     P = len(t)
     n_val = max(1, int(P * val_fraction))
     n_train = P - n_val
     net = Net(n, batch)
-    net.set_params(init_params(init_seed))
+    net.set_params(init_params(init_seed) if init is None else np.asarray(init, np.float32))
     net.set_dataset(x, t)
     v0 = net.eval(n_train, n_val)
     s1 = int(steps * 0.7)
@@ -269,6 +279,8 @@ def main(argv=None):
     tr.add_argument("--train-steps", type=int, default=4000)
     tr.add_argument("--batch", type=int, default=8)
     tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--scenario", choices=("flume", "beach"), default="flume")
+    tr.add_argument("--init", default=None, help="weights to fine-tune from")
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=WEIGHTS)
     ev.add_argument("--out", required=True)
@@ -296,12 +308,14 @@ def main(argv=None):
         return 0
     if args.cmd == "train":
         t0 = time.time()
-        x, t, meta = build_dataset(args.runs, args.steps, args.every)
-        n = run_settings(0.08, 1.4).n
-        params, info = fit(x, t, n, args.batch, args.train_steps, args.lr)
+        x, t, meta = build_dataset(args.runs, args.steps, args.every, scenario=args.scenario)
+        n = (beach_settings(0.08, 1.4) if args.scenario == "beach" else run_settings(0.08, 1.4)).n
+        init = np.load(args.init) if args.init else None
+        params, info = fit(x, t, n, args.batch, args.train_steps, args.lr, init=init)
         os.makedirs(os.path.dirname(args.out), exist_ok=True)
         np.save(args.out, params)
-        info.update({"runs": meta, "pairs": int(len(t)), "grid": list(n), "seconds": time.time() - t0})
+        info.update({"runs": meta, "pairs": int(len(t)), "grid": list(n), "scenario": args.scenario,
+                     "init": os.path.basename(args.init) if args.init else None, "seconds": time.time() - t0})
         with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
             json.dump(info, f, indent=1)
         print(json.dumps({k: v for k, v in info.items() if k != "loss_curve"}))

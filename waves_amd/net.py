@@ -120,6 +120,29 @@ class Net:
                                            ctypes.c_uint64(int(seed)), self._f(losses)))
         return losses
 
+    def layers(self):
+        """[dict(level, cells, coarse_cells, ca, cb, cout)] of every layer in
+        forward order: ca input channels at the layer's level, cb upsampled
+        from the next level (coarse_cells), cout outputs."""
+        dims = level_dims(self.n) + [(0, 0, 0)]
+        out = []
+        for l in range(int(self._lib.wv_net_layer_count())):
+            v = [ctypes.c_int() for _ in range(4)]
+            self._check(self._lib.wv_net_layer_info(l, *[ctypes.byref(x) for x in v]))
+            lv, ca, cb, co = (x.value for x in v)
+            d, dc = dims[lv], dims[lv + 1]
+            out.append({"level": lv, "cells": d[0] * d[1] * d[2] * self.batch,
+                        "coarse_cells": dc[0] * dc[1] * dc[2] * self.batch, "ca": ca, "cb": cb, "cout": co})
+        return out
+
+    def time_layers(self, reps: int = 20) -> np.ndarray:
+        """Mean device microseconds per launch of every layer's convolution
+        (inputs from one forward pass of the batch in the input buffer)."""
+        n = int(self._lib.wv_net_layer_count())
+        us = np.zeros(n, np.float32)
+        self._check(self._lib.wv_net_time_layers(self._h, int(reps), self._f(us), n))
+        return us
+
     def eval(self, first: int, count: int) -> float:
         loss = np.zeros(1, np.float32)
         self._check(self._lib.wv_net_eval(self._h, int(first), int(count), self._f(loss)))

@@ -39,7 +39,8 @@ EXPORTS = ("wv_create", "wv_destroy", "wv_reset", "wv_request_steps", "wv_reques
 #: every entry point declared in include/waves_net.h
 NET_EXPORTS = ("wv_net_param_count", "wv_net_create", "wv_net_destroy", "wv_net_set_params",
                "wv_net_get_params", "wv_net_forward", "wv_net_loss_grad", "wv_net_set_dataset",
-               "wv_net_train", "wv_net_eval", "wv_net_last_error", "wv_solver_attach_net")
+               "wv_net_train", "wv_net_eval", "wv_net_last_error", "wv_solver_attach_net",
+               "wv_net_layer_count", "wv_net_layer_info", "wv_net_time_layers")
 SOLVE_ZERO_MEAN, SOLVE_GIVEN_RHS = 1, 2
 
 
@@ -126,6 +127,9 @@ def library() -> ctypes.CDLL:
         L.wv_net_set_params.argtypes = [vp, fp, i64]
         L.wv_net_get_params.argtypes = [vp, fp, i64]
         L.wv_net_forward.argtypes = [vp, u16p, fp, fp]
+        L.wv_net_layer_count.argtypes = []
+        L.wv_net_layer_info.argtypes = [i32] + [ctypes.POINTER(i32)] * 4
+        L.wv_net_time_layers.argtypes = [vp, i32, fp, i32]
         L.wv_net_loss_grad.argtypes = [vp, u16p, fp, fp, fp]
         L.wv_net_set_dataset.argtypes = [vp, u16p, fp, i32]
         L.wv_net_train.argtypes = [vp, i32, i32, ctypes.c_float, ctypes.c_uint64, fp]
