@@ -63,6 +63,7 @@ struct ConvArgs {
   bf16_t* yb; float* yf;
   Grid g, gb;
   int ca, cb, kp, cout, ystride, bias, relu;
+  int groups;   // k_conv_tiled: output-channel groups of NT x 16 channels, one block each (0 or 1: one group)
 };
 
 // y[c][co] = act(sum_k A[c][k] W[co][k]); A[c][t cin + ci] = input ci at the
@@ -150,9 +151,10 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lds[];
   bf16_t* halo = lds;                                   // HALO * CIN
   bf16_t* wl = lds + HALO * CIN;                        // NT*16 rows x kp (WLDS)
-  const int tilesz = (a.g.nz + TZ - 1) / TZ;
+  const int tilesz = (a.g.nz + TZ - 1) / TZ, groups = a.groups > 1 ? a.groups : 1;
+  const int zb = (int)blockIdx.z / groups, co0 = ((int)blockIdx.z - zb * groups) * NT * 16;   // this block's channels
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-  const int z0 = (blockIdx.z % tilesz) * TZ, b = blockIdx.z / tilesz;
+  const int z0 = (zb % tilesz) * TZ, b = zb / tilesz;
   constexpr int Q = CIN / 4;                            // 8-byte chunks per halo cell
   constexpr int TOTAL = HALO * Q, PER = 8;              // PER loads in flight per thread
   for (int base = 0; base < TOTAL; base += 256 * PER) {
@@ -184,7 +186,7 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
       for (int u = 0; u < PER; u++) {
         const int e = base + u * 256 + threadIdx.x;
         const int row = e / kq4, k = (e - row * kq4) * 4;
-        v[u] = (e < wtotal && row < a.cout) ? *(const v4s*)(a.w + (int64_t)row * a.kp + k) : v4s{0, 0, 0, 0};
+        v[u] = (e < wtotal && co0 + row < a.cout) ? *(const v4s*)(a.w + (int64_t)(co0 + row) * a.kp + k) : v4s{0, 0, 0, 0};
       }
 #pragma unroll
       for (int u = 0; u < PER; u++) {
@@ -212,8 +214,8 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
     const int k = k0 + 8 * kq;
     v8s bv[NT];
     for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + m;
-      if (WLDS) bv[t] = *(const v8s*)(wl + co * a.kp + k);
+      const int cl = t * 16 + m, co = co0 + cl;
+      if (WLDS) bv[t] = *(const v8s*)(wl + cl * a.kp + k);
       else bv[t] = co < a.cout ? *(const v8s*)(a.w + (int64_t)co * a.kp + k) : v8s{0, 0, 0, 0, 0, 0, 0, 0};
     }
     const int off = tap_offset(k);
@@ -228,8 +230,8 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
     const int k = k0 + 4 * kq;
     v4s bv[NT];
     for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + m;
-      if (WLDS) bv[t] = *(const v4s*)(wl + co * a.kp + k);
+      const int cl = t * 16 + m, co = co0 + cl;
+      if (WLDS) bv[t] = *(const v4s*)(wl + cl * a.kp + k);
       else bv[t] = co < a.cout ? *(const v4s*)(a.w + (int64_t)co * a.kp + k) : v4s{0, 0, 0, 0};
     }
     const int off = tap_offset(k);
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__(256) k_conv_tiled(ConvArgs a) {
     if (y >= a.g.ny || z >= a.g.nz) continue;
     for (int t = 0; t < NT; t++)
       for (int j = 0; j < 4; j++) {
-        const int x = x0 + 4 * kq + j, co = t * 16 + m;
+        const int x = x0 + 4 * kq + j, co = co0 + t * 16 + m;
         if (x < a.g.nx && co < a.cout) {
           float v = acc[r][t][j];
           if (a.relu) v = v > 0.f ? v : 0.f;
@@ -270,7 +272,7 @@ static int lds_limit() {
 
 template <int CIN, int NT>
 static bool launch_tiled_nt(const ConvArgs& a, hipStream_t st) {
-  const dim3 grid((a.g.nx + TX - 1) / TX, (a.g.ny + TY - 1) / TY, ((a.g.nz + TZ - 1) / TZ) * a.g.B);
+  const dim3 grid((a.g.nx + TX - 1) / TX, (a.g.ny + TY - 1) / TY, ((a.g.nz + TZ - 1) / TZ) * a.g.B * (a.groups > 1 ? a.groups : 1));
   const size_t halo = (size_t)HALO * CIN * 2, wts = (size_t)NT * 16 * a.kp * 2;
   const size_t lim = (size_t)lds_limit();
   // weights join the halo in LDS on small grids, or where the block still fits
@@ -287,9 +289,16 @@ static bool launch_tiled_nt(const ConvArgs& a, hipStream_t st) {
   }
   return false;
 }
+// A grid of fewer spatial tiles than twice the CUs gets one block per
+// 16-channel output tile (the input halo is then read once per tile).
 template <int CIN>
-static bool launch_tiled_cin(const ConvArgs& a, hipStream_t st) {
-  switch ((a.cout + 15) / 16) {
+static bool launch_tiled_cin(const ConvArgs& a0, hipStream_t st) {
+  ConvArgs a = a0;
+  const int tiles = (a.cout + 15) / 16;
+  const int64_t spatial = (int64_t)((a.g.nx + TX - 1) / TX) * ((a.g.ny + TY - 1) / TY) * ((a.g.nz + TZ - 1) / TZ) * a.g.B;
+  if (tiles > 1 && spatial < 512) { a.groups = tiles; return launch_tiled_nt<CIN, 1>(a, st); }
+  a.groups = 1;
+  switch (tiles) {
   case 1: return launch_tiled_nt<CIN, 1>(a, st);
   case 2: return launch_tiled_nt<CIN, 2>(a, st);
   case 4: return launch_tiled_nt<CIN, 4>(a, st);
