@@ -9,7 +9,6 @@ time) is identical."""
 import numpy as np
 import pytest
 
-import bench
 from waves_amd import solver as S
 from waves_amd.settings import narrow_flume
 
@@ -20,10 +19,10 @@ def advance(prec, steps, warm=20):
     st = narrow_flume()
     g = S.Solver(st)
     g.start()
-    bench.run_steps(g, warm)          # same plain start for both
+    g.run_steps(warm)                 # same plain start for both
     g.synchronize()
     g.set_precision(prec)
-    c, _ = bench.run_steps(g, steps)
+    g.run_steps(steps)
     g.synchronize()
     c = g.poll(wait=True)
     F = g.read(int(c[0]))[:st.cells].copy()
