@@ -261,11 +261,14 @@ def refined_leg(steps: int = 20):
     return out
 
 
-def full_run(t_end: float = FULL_RUN):
-    """The whole 20 sim-s run from rest, timed (reset excluded)."""
-    s = benchmark_beach()
+def full_run(t_end: float = FULL_RUN, tol=None, precision: int = 0):
+    """The whole 20 sim-s run from rest, timed (reset excluded); by default
+    at the benchmark's PCG tolerance in plain FP32, else at `tol` and with
+    the batch program's PCG in `precision` (1: BF16 V-cycle, FP64 dots)."""
+    s = benchmark_beach() if tol is None else benchmark_beach(pressure_tolerance=tol)
     g = Solver(s)
     g.start()
+    g.set_precision(precision)
     t0 = time.perf_counter()
     curve = []
     c = run_to_time(g, t_end, curve=curve)
@@ -279,7 +282,9 @@ def full_run(t_end: float = FULL_RUN):
            "rejections": int(c[3]), "pcg_iterations": int(c[18]), "failure": int(c[7]),
            "submissions": st["submissions"], "readbacks_per_accepted_step": st["readbacks"] / max(int(c[2]), 1),
            "gpu_s": st["gpu_ms"] / 1e3, "R_fluid": st["r_fluid"], "pressure": pressure_ms(rows),
-           "wall_at_sim_second": curve}
+           "wall_at_sim_second": curve, "pcg_tolerance": s.pressure_tolerance,
+           "pcg_form": "mixed_bf16_fp64" if precision else "plain_fp32",
+           "completed": bool(float(c[1]) >= t_end and int(c[7]) == 0)}
     g.close()
     return out
 
@@ -298,6 +303,9 @@ def main():
     ap.add_argument("--save-state", help="write the state at the window start to this state file")
     ap.add_argument("--load-state", help="start the window from this state file instead of the untimed run")
     ap.add_argument("--progress", action="store_true", help="print each simulated second reached to stderr")
+    ap.add_argument("--full-tol", type=float, default=None, help="PCG tolerance of the full run (default: the benchmark's)")
+    ap.add_argument("--full-precision", type=int, default=0, choices=(0, 1),
+                    help="PCG of the full run's batch program: 0 plain FP32, 1 mixed (BF16 V-cycle, FP64 dots)")
     args = ap.parse_args()
     global PROGRESS
     PROGRESS = args.progress
@@ -401,7 +409,7 @@ def main():
     }
     solver.close()
     if not args.no_full:
-        result["full_run"] = full_run()
+        result["full_run"] = full_run(tol=args.full_tol, precision=args.full_precision)
     if not args.no_pressure:
         result["pressure"] = pressure_leg()
     if not args.no_extra:

@@ -49,6 +49,7 @@ def lib():
         L.wo_mg_record.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.wo_initial.argtypes = [vp]
         L.wo_set_quota.argtypes = [vp, ctypes.c_float]
+        L.wo_set_scatter_order.argtypes = [vp, ctypes.c_int]
         L.wo_batch.argtypes = [vp, ctypes.c_float]
         L.wo_levels.argtypes = [vp]
         L.wo_level_meta.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
@@ -111,6 +112,10 @@ class Oracle:
 
     def set_quota(self, quota: float):
         lib().wo_set_quota(self.h, quota)
+
+    def set_scatter_order(self, reverse: bool):
+        """Galerkin scatter additions in descending (True) or ascending entry order."""
+        lib().wo_set_scatter_order(self.h, int(bool(reverse)))
 
     def initial(self):
         lib().wo_initial(self.h)

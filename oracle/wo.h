@@ -48,6 +48,12 @@ int wo_set(WO *s, int field, const void *src, int64_t count);
 void wo_op(WO *s, int op, int64_t groups);
 void wo_mg_record(WO *s, int build, int initial);
 void wo_set_quota(WO *s, float quota);
+/* order of the Galerkin scatter's additions (MG op 3): 0 ascending entry
+ * index, 1 descending.  The reference adds with float compare-and-swap
+ * atomics in no fixed order (pressure_multigrid.glslinc:128-134), so both are
+ * executions of the reference; the switch measures what the order alone does
+ * to a trajectory. */
+void wo_set_scatter_order(WO *s, int reverse);
 /* initial command list: ops 0,40,(perturb),17,18,16,19,41,42,49,85,86 */
 void wo_initial(WO *s);
 /* one full batch of water/coupled_gpu.gd:38-56 + coastal ops, quota as in

@@ -73,6 +73,7 @@ int wo_set(WO *s, int field, const void *src, int64_t count) {
 }
 
 void wo_set_quota(WO *s, float quota) { s->pc[15] = quota; }
+void wo_set_scatter_order(WO *s, int reverse) { s->scatter_reverse = reverse != 0; }
 
 /* coastal.glsl:15 routing; op 90 is the benchmark perturbation (no reference op) */
 void wo_op(WO *s, int op, int64_t groups) {

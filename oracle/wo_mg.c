@@ -174,7 +174,9 @@ void wo_mg_op(WO *s, int op, int l, int phase, int field, int64_t groups) {
   case 2: memset(s->matrix + md(s, l, 4), 0, (size_t)N * 125 * 4); return;
   case 3: { /* pressure_multigrid.glslinc:159-175 */
     int width = md(s, l, 11);
-    for (int64_t i = 0; i < (int64_t)N * width; i++) {
+    const int64_t entries = (int64_t)N * width;
+    for (int64_t e = 0; e < entries; e++) {
+      const int64_t i = s->scatter_reverse ? entries - 1 - e : e;
       int row = (int)(i / width), k = (int)(i % width);
       if (!active_cell(s, l, row)) continue;
       I3 q = coord(row, nl), off = moffset(s, l, k);

@@ -54,6 +54,7 @@ struct WO {
   int (*build)[4]; int nbuild;
   int mg_phase;            /* run.z of the current MG dispatch */
   int mg_run[4];
+  int scatter_reverse;     /* Galerkin scatter (MG op 3) in descending entry order */
 };
 
 static inline I3 i3(int x, int y, int z) { I3 r; r.v[0] = x; r.v[1] = y; r.v[2] = z; return r; }
