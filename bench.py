@@ -196,7 +196,7 @@ def net_layer_roofline(net, reps: int = 20):
     level, cb at the next coarser level, bf16), the output written once (bf16
     after a ReLU, fp32 for the two heads) and the bf16 weights.  Times are HIP
     events around `reps` launches of the layer on the network's stream."""
-    us = net.time_layers(reps)
+    us = [float(v) for v in net.time_layers(reps)]
     rows, tot_f, tot_us = [], 0.0, 0.0
     names = ("e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b3a", "b3b", "d2a", "d2b", "hc", "d1a", "d1b", "d0a", "d0b", "hf")
     for name, L, t in zip(names, net.layers(), us):
