@@ -276,6 +276,7 @@ def main(argv=None):
     tr.add_argument("--runs", type=int, default=8)
     tr.add_argument("--steps", type=int, default=1000)
     tr.add_argument("--every", type=int, default=4)
+    tr.add_argument("--skip", type=int, default=0, help="accepted steps of each run before the first capture")
     tr.add_argument("--train-steps", type=int, default=4000)
     tr.add_argument("--batch", type=int, default=8)
     tr.add_argument("--lr", type=float, default=1e-3)
@@ -308,7 +309,7 @@ def main(argv=None):
         return 0
     if args.cmd == "train":
         t0 = time.time()
-        x, t, meta = build_dataset(args.runs, args.steps, args.every, scenario=args.scenario)
+        x, t, meta = build_dataset(args.runs, args.steps, args.every, skip=args.skip, scenario=args.scenario)
         n = (beach_settings(0.08, 1.4) if args.scenario == "beach" else run_settings(0.08, 1.4)).n
         init = np.load(args.init) if args.init else None
         params, info = fit(x, t, n, args.batch, args.train_steps, args.lr, init=init)
