@@ -829,6 +829,15 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int 
     wsup[k] = wI; isup[k] = i;
   }
   __syncthreads();
+  // the support's (A P) boxes, read once with 16-byte loads (a box is at most
+  // 64 floats, contiguous and padded to 4); the entries are summed from LDS
+  const int Ep = (F.box[0] * F.box[1] * F.box[2] + 3) & ~3, q4 = Ep >> 2;
+  __shared__ float4 sbox[GATHER_SUPPORT * 16];
+  for (int e = k; e < total * q4; e += (int)blockDim.x) {
+    const int c = e / q4, r = e - c * q4;
+    sbox[c * q4 + r] = wsup[c] != 0 ? reinterpret_cast<const float4*>(m.ap + (int64_t)isup[c] * Ep)[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  __syncthreads();
   if (k >= W) return;
   const I3 off = moffset(m, lc, k);
   int J[3] = {qc.x + off.x, qc.y + off.y, qc.z + off.z};
@@ -841,11 +850,12 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int 
     if (m.perx) d.x = pmod(d.x + nc.x / 2, nc.x) - nc.x / 2;
     d.z = pmod(d.z + nc.z / 2, nc.z) - nc.z / 2;
     if (d.x == off.x && d.y == off.y && d.z == off.z) {
-      const int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2], Ep = (E0 * E1 * E2 + 3) & ~3;
+      const int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2];
+      const float* boxes = reinterpret_cast<const float*>(sbox);
       for (int c = 0; c < total; c++) {
         const float w = wsup[c]; if (w == 0) continue;
         const int ax = asup[c][0], ay = asup[c][1], az = asup[c][2];
-        const float* T = m.ap + (int64_t)isup[c] * Ep;
+        const float* T = boxes + c * Ep;
         // box ranges whose positions map to J along each axis
         int ylo = J[1] == 0 ? 0 : J[1] - ay, yhi = J[1] - ay;
         if (ylo < 0) ylo = 0; if (yhi > E1 - 1) yhi = E1 - 1;

@@ -7,6 +7,8 @@
 
 __device__ __forceinline__ int pmod(int x, int n) {
   if ((unsigned)x < (unsigned)n) return x;   // in range: the modulo is the identity
+  const int y = x < 0 ? x + n : x - n;       // one period away: one add (almost every wrap of a stencil or transfer offset)
+  if ((unsigned)y < (unsigned)n) return y;
   return x >= 0 ? x % n : (n - 1) - ((-1 - x) % n);
 }
 __device__ __forceinline__ int64_t at(const MGDev& m, int l, int field, int64_t i) { return m.lv[l].vec_off + (int64_t)field * m.lv[l].count + i; }
