@@ -12,7 +12,11 @@ synchronisation.  Submissions follow the reference's scheduler
 (water/work_scheduler.gd): at most three in flight, one control readback each,
 the quota stops the device after step K.  `value` is the timed window's
 accepted sim-s / wall-s; `full_run` is the whole 20 sim-s run from rest, timed
-in the same process.
+in the same process, with its rate over the breaking phase (6 to 12 sim-s);
+`window_scan` is the distribution of the same metric over consecutive windows
+of K steps from 6 to 12 sim-s.  A step's simulated advance is set by the CFL
+limit and some trials are rejected, so a short window varies several-fold
+with where it starts (K = 20: p10 0.19, median 0.37, p90 0.61).
 
 Also reported: metric 2, the in-loop pressure time per accepted step (device
 clock around the pressure sections: hierarchy audit and rebuild, PCG,
@@ -261,6 +265,46 @@ def refined_leg(steps: int = 20):
     return out
 
 
+def phase_rate(curve, t0: float, t1: float):
+    """Accepted sim-s per wall-s between the first whole-second marks at or
+    after t0 and t1 of a run's (wall s, sim-s) curve."""
+    a = next((p for p in curve if p[1] >= t0), None)
+    b = next((p for p in curve if p[1] >= t1), None)
+    if a is None or b is None or b[0] <= a[0]:
+        return None
+    return (b[1] - a[1]) / (b[0] - a[0])
+
+
+def scan_windows(steps: int, t0: float = 6.0, t1: float = 12.0):
+    """Consecutive windows of `steps` accepted steps from t0 (after an
+    untimed run from rest) until t1, each timed like the bench's window:
+    the distribution of the windowed metric over the breaking phase."""
+    s = benchmark_beach()
+    g = Solver(s)
+    g.start()
+    run_to_time(g, t0)
+    g.synchronize()
+    vals = []
+    while True:
+        c = g.poll(wait=True)
+        ts = float(c[1])
+        w0 = time.perf_counter()
+        c = g.run_steps(steps)
+        g.synchronize()
+        wall = time.perf_counter() - w0
+        c = g.poll(wait=True)
+        vals.append((float(c[1]) - ts) / wall)
+        if PROGRESS and len(vals) % 25 == 0:
+            print(f"{len(vals)} windows, t {float(c[1]):.3f} s", file=sys.stderr, flush=True)
+        if float(c[1]) >= t1 or stopped(c):
+            break
+    g.close()
+    v = np.array(vals)
+    return {"steps_per_window": steps, "from_s": t0, "to_s": t1, "windows": len(vals), "median": float(np.median(v)),
+            "p10": float(np.percentile(v, 10)), "p90": float(np.percentile(v, 90)), "min": float(v.min()),
+            "max": float(v.max()), "mean": float(v.mean())}
+
+
 def full_run(t_end: float = FULL_RUN, tol=None, precision: int = 0):
     """The whole 20 sim-s run from rest, timed (reset excluded); by default
     at the benchmark's PCG tolerance in plain FP32, else at `tol` and with
@@ -282,7 +326,8 @@ def full_run(t_end: float = FULL_RUN, tol=None, precision: int = 0):
            "rejections": int(c[3]), "pcg_iterations": int(c[18]), "failure": int(c[7]),
            "submissions": st["submissions"], "readbacks_per_accepted_step": st["readbacks"] / max(int(c[2]), 1),
            "gpu_s": st["gpu_ms"] / 1e3, "R_fluid": st["r_fluid"], "pressure": pressure_ms(rows),
-           "wall_at_sim_second": curve, "pcg_tolerance": s.pressure_tolerance,
+           "wall_at_sim_second": curve, "breaking_phase_6_12s": phase_rate(curve, 6.0, 12.0),
+           "pcg_tolerance": s.pressure_tolerance,
            "pcg_form": "mixed_bf16_fp64" if precision else "plain_fp32",
            "completed": bool(float(c[1]) >= t_end and int(c[7]) == 0)}
     g.close()
@@ -303,6 +348,8 @@ def main():
     ap.add_argument("--save-state", help="write the state at the window start to this state file")
     ap.add_argument("--load-state", help="start the window from this state file instead of the untimed run")
     ap.add_argument("--progress", action="store_true", help="print each simulated second reached to stderr")
+    ap.add_argument("--no-scan", action="store_true",
+                    help="skip the distribution of --steps-step windows from 6 to 12 sim-s (run with the full run)")
     ap.add_argument("--full-tol", type=float, default=None, help="PCG tolerance of the full run (default: the benchmark's)")
     ap.add_argument("--full-precision", type=int, default=0, choices=(0, 1),
                     help="PCG of the full run's batch program: 0 plain FP32, 1 mixed (BF16 V-cycle, FP64 dots)")
@@ -410,6 +457,8 @@ def main():
     solver.close()
     if not args.no_full:
         result["full_run"] = full_run(tol=args.full_tol, precision=args.full_precision)
+    if not args.no_full and not args.no_scan:
+        result["window_scan"] = scan_windows(args.steps)
     if not args.no_pressure:
         result["pressure"] = pressure_leg()
     if not args.no_extra:
