@@ -704,13 +704,16 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l, i
   }
   }
   __syncthreads();
-  if (!live || lane != 0) return;
-  auto box = [&](int b) { if (b >= E) return 0.f; float v = acc[b * 128 + tid]; for (int g = 1; g < G; g++) v += acc[b * 128 + tid + g]; return v; };
+  if (!live) return;
+  // entry b of the row folds its G lane slices in lane order; the row's lanes
+  // share the entries (lane g: quads g, g + G, ...)
+  const int t0 = tid - lane;
+  auto box = [&](int b) { if (b >= E) return 0.f; float v = acc[b * 128 + t0]; for (int g = 1; g < G; g++) v += acc[b * 128 + t0 + g]; return v; };
   // (A P)_i is stored row-major, padded to a multiple of 4 floats, so that the
   // second stage reads a fine row's box with 16-byte loads
   const int Ep = (E + 3) & ~3;
   float4* T = (float4*)(m.ap + i * Ep);
-  for (int b = 0; b < Ep; b += 4) T[b >> 2] = make_float4(box(b), box(b + 1), box(b + 2), box(b + 3));
+  for (int b = 4 * lane; b < Ep; b += 4 * G) T[b >> 2] = make_float4(box(b), box(b + 1), box(b + 2), box(b + 3));
 }
 #define GAL_THREADS 64
 __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m, int l) {
@@ -776,11 +779,14 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
     }
   }
   __syncthreads();
-  if (I >= C.count || lane != 0) return;
+  if (I >= C.count) return;
+  // entry k folds the node's G lane slices in lane order; the node's lanes
+  // share the entries (lane g: entries g, g + G, ...)
+  const int t0 = tid - lane;
   float* row = m.rows + C.row_off + I;
-  for (int k = 0; k < W; k++) {
+  for (int k = lane; k < W; k += G) {
     float v;
-    if (live) { v = 0; for (int g = 0; g < G; g++) v += acc[k * GAL_THREADS + tid + g]; }
+    if (live) { v = 0; for (int g = 0; g < G; g++) v += acc[k * GAL_THREADS + t0 + g]; }
     else { I3 off = moffset(m, lc, k); v = (off.x == 0 && off.y == 0 && off.z == 0) ? 1.0f : 0.0f; }
     row[(int64_t)k * C.count] = v;
   }
