@@ -49,7 +49,10 @@ void wv_launch_reflect_table(const Ctx& s, hipStream_t st);  // once per solver:
 void wv_launch_press_begin(const Ctx& s, hipStream_t st);
 void wv_launch_press_end(const Ctx& s, hipStream_t st);
 void wv_launch_step_record(const Ctx& s, hipStream_t st);
-void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st);
+// op 8, with the level-0 load of the V-cycle that follows it (and that
+// V-cycle's first sweep when wv_mg_first_sweep_fused)
+void wv_launch_pcg_update(const Ctx& s, const MGDev& m, hipStream_t st);
+bool wv_mg_first_sweep_fused(const MGDev& m);
 void wv_launch_spmv_dot(const Ctx& s, hipStream_t st);  // ops 5 + 6 + 7 (plain FP32 form)
 // Multigrid: build = hierarchy audit + Galerkin rebuild (phase 2);
 // cycle = one V-cycle (phase 0 initial / 1 iteration) ending in the fused

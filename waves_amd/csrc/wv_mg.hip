@@ -18,11 +18,20 @@
 #include "wv_mgdev.h"
 
 // ------------------------------------------------------------ V-cycle -------
-// pressure_multigrid.glslinc:191 (op 5): b = r, x1 = x2 = 0 on level 0
-__global__ void __launch_bounds__(256) k_mg_load(Ctx s, MGDev m, int phase) {
+// Levels whose pre-smoothing sweep 1 is done by the kernel that loads their b
+// (cheb_first): every level that is smoothed, i.e. all but the last.
+static inline bool first_sweep_fused(const MGDev& m, int l) { return l < m.L - 1; }
+bool wv_mg_first_sweep_fused(const MGDev& m) { return first_sweep_fused(m, 0); }
+
+// pressure_multigrid.glslinc:191 (op 5): b = r, x1 = x2 = 0 on level 0, and
+// with `fuse` the level's first Chebyshev sweep from that x
+__global__ void __launch_bounds__(256) k_mg_load(Ctx s, MGDev m, int phase, int fuse) {
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[0].count) return;
-  m.vec[at(m, 0, 0, i)] = s.work[i]; m.vec[at(m, 0, 1, i)] = 0; m.vec[at(m, 0, 2, i)] = 0;
+  const float b = s.work[i];
+  m.vec[at(m, 0, 0, i)] = b; m.vec[at(m, 0, 1, i)] = 0;
+  if (fuse) cheb_first(&m.vec[at(m, 0, 7, i)], &m.vec[at(m, 0, 2, i)], m.vec[at(m, 0, 4, i)], b);
+  else m.vec[at(m, 0, 2, i)] = 0;
 }
 // pressure_multigrid.glslinc:192-196: 4th-kind Chebyshev sweep k, l1-Jacobi scaled
 template <bool H>
@@ -30,7 +39,7 @@ __global__ void __launch_bounds__(256) k_mg_cheb(Ctx s, MGDev m, int l, int sour
   if (!mg_enabled(s, phase)) return;
   int64_t i = GID(); if (i >= m.lv[l].count) return;
   float r = m.vec[at(m, l, 0, i)] - mapply0<H>(m, source, (int)i);
-  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * m.vec[at(m, l, 7, i)]) + (float)(8 * k - 4) / (float)(2 * k + 1) * m.vec[at(m, l, 4, i)] * r;
+  float d = cheb_dir(k, m.vec[at(m, l, 7, i)], m.vec[at(m, l, 4, i)], r);
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = m.vec[at(m, l, source, i)] + d;
 }
 // ---- coarse levels (l >= 1): G lanes per row --------------------------------
@@ -151,7 +160,7 @@ __device__ __forceinline__ void epilogue_store(const MGDev& m, int l, int source
   float r = e.b - ax;
   if (OP == 1) { m.vec[at(m, l, 6, i)] = r; return; }
   if (OP == 2) { m.vec[at(m, l, 3 - source, i)] = e.xs + e.dinv * r; return; }
-  float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * e.dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * e.dinv * r;
+  float d = cheb_dir(k, e.dprev, e.dinv, r);
   m.vec[at(m, l, 7, i)] = d; m.vec[at(m, l, 3 - source, i)] = e.xs + d;
 }
 template <int OP, bool H, int WC>
@@ -232,7 +241,9 @@ __global__ void __launch_bounds__(256) k_mg_coarse(Ctx s, MGDev m, int l, int so
 // per LDS read, so the sum is bit-identical to one thread walking the support.
 #define MG_RESTRICT_MAX 64      // children of a node: a 4x4x4 (trilinear) or 2x2x2 (aggregation) box
 #define MG_RESTRICT_STRIDE 68   // slice stride: float4-aligned, successive nodes four banks apart
-__global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase) {
+// With `fuse` the node's first Chebyshev sweep follows from the b just summed
+// (cheb_first).
+__global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l, int phase, int fuse) {
   if (!mg_enabled(s, phase)) return;   // uniform over the launch: no thread reaches a barrier
   extern __shared__ float part[];      // (256 / G) node slices when G > 1
   const MGLevel& C = m.lv[l]; int parent = l - 1; const MGLevel& P = m.lv[parent];
@@ -290,7 +301,9 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
     }
   }
   if (!rc.valid || rc.lane != 0) return;
-  m.vec[at(m, l, 0, rc.row)] = sum; m.vec[at(m, l, 1, rc.row)] = 0; m.vec[at(m, l, 2, rc.row)] = 0;
+  m.vec[at(m, l, 0, rc.row)] = sum; m.vec[at(m, l, 1, rc.row)] = 0;
+  if (fuse) cheb_first(&m.vec[at(m, l, 7, rc.row)], &m.vec[at(m, l, 2, rc.row)], m.vec[at(m, l, 4, rc.row)], sum);
+  else m.vec[at(m, l, 2, rc.row)] = 0;
 }
 static size_t restrict_lds(const MGDev& m, int l) {
   const int G = m.lv[l].rlanes;
@@ -1103,7 +1116,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
       float ax = apply(l, source, i), xs = vec(l, source)[i];
       if (!active(l, i)) ax = xs;
       float r = vec(l, 0)[i] - ax;
-      float d = (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * vec(l, 7)[i]) + (float)(8 * k - 4) / (float)(2 * k + 1) * vec(l, 4)[i] * r;
+      float d = cheb_dir(k, vec(l, 7)[i], vec(l, 4)[i], r);
       vec(l, 7)[i] = d; vec(l, 3 - source)[i] = xs + d;
     }
     __syncthreads();
@@ -1154,7 +1167,9 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
           sum += weight * vec(parent, 6)[j];
         }
       }
-      vec(lc, 0)[I] = sum; vec(lc, 1)[I] = 0; vec(lc, 2)[I] = 0;
+      vec(lc, 0)[I] = sum; vec(lc, 1)[I] = 0;
+      if (lc < tl.L - 1) cheb_first(&vec(lc, 7)[I], &vec(lc, 2)[I], vec(lc, 4)[I], sum);
+      else vec(lc, 2)[I] = 0;
     }
     __syncthreads();
   };
@@ -1185,10 +1200,13 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
     }
     __syncthreads();
   };
+  // every level here is smoothed (the last one is solved), so the kernel that
+  // set its b -- the restriction launch into level T, restrict_to below it --
+  // did its first sweep (cheb_first)
   int fields[WV_MAXL];
   for (int l = tl.T; l < tl.L - 1; l++) {
-    int f = 1;
-    for (int k = 1; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
+    int f = 2;
+    for (int k = 2; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
     fields[l] = f;
     residual(l, f);
     restrict_to(l + 1);
@@ -1238,11 +1256,12 @@ template <bool H>
 struct CycleBuilder {
   const Ctx& s; const MGDev& m; int phase, smoothing; hipStream_t st;
   unsigned coop_blocks(int l) { return nblocks((int64_t)m.lv[l].count * m.lv[l].lanes, 256); }
-  int smooth(int l, int field) {
-    for (int sweep = 0; sweep < smoothing; sweep++) {
-      if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, sweep + 1, phase);
-      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, sweep + 1, phase);
-      else launch_coarse<0, H>(s, m, l, field, sweep + 1, phase, st);
+  // sweeps k0..smoothing of level l from `field`; returns the field holding x
+  int smooth(int l, int field, int k0) {
+    for (int k = k0; k <= smoothing; k++) {
+      if (l == 0) hipLaunchKernelGGL(k_mg_cheb<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, k, phase);
+      else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<0, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, k, phase);
+      else launch_coarse<0, H>(s, m, l, field, k, phase, st);
       field = 3 - field;
     }
     return field;
@@ -1264,19 +1283,22 @@ struct CycleBuilder {
       }
       return tf;
     }
-    int field = smooth(l, 1);
+    // sweep 1 was done by the kernel that set this level's b: the V-cycle's
+    // load or the PCG update (level 0), the restriction (below)
+    int field = smooth(l, 2, 2);
     if (l == 0) hipLaunchKernelGGL(k_mg_residual<H>, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, phase);
     else if (m.lv[l].width > 64) hipLaunchKernelGGL((k_mg_coarse_wide<1, H, 8>), dim3(nblocks(m.lv[l].count, 64)), dim3(512), 0, st, s, m, l, field, 0, phase);
     else launch_coarse<1, H>(s, m, l, field, 0, phase, st);
-    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), restrict_lds(m, l + 1), st, s, m, l + 1, phase);
+    hipLaunchKernelGGL(k_mg_restrict_coop, dim3(nblocks((int64_t)m.lv[l + 1].count * m.lv[l + 1].rlanes, 256)), dim3(256), restrict_lds(m, l + 1), st, s, m, l + 1, phase,
+                       (int)first_sweep_fused(m, l + 1));
     int cf = cycle(l + 1);
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(m.lv[l].count, 256)), dim3(256), 0, st, s, m, l, field, cf, phase);
-    return smooth(l, field);
+    return smooth(l, field, 1);
   }
 };
 
 void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st) {
-  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
+  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase, (int)first_sweep_fused(m, 0));
   CycleBuilder<false> b{s, m, phase, smoothing, st};
   int final_field = b.cycle(0);
   hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
@@ -1351,7 +1373,7 @@ void wv_mg_to_bf16(const Ctx& s, const MGDev& m, hipStream_t st) {
 }
 void wv_launch_mg_out64(const Ctx& s, const MGDev& m, int field, int phase, hipStream_t st);
 void wv_mg_cycle_bf16(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, bool fp64_out, hipStream_t st) {
-  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase);
+  if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase, (int)first_sweep_fused(m, 0));
   CycleBuilder<true> b{s, m, phase, smoothing, st};
   int final_field = b.cycle(0);
   if (fp64_out) wv_launch_mg_out64(s, m, final_field, phase, st);

@@ -75,6 +75,22 @@ __device__ __forceinline__ int mtransfers(const MGDev& m, int l, int i, int* ids
   }
   return size;
 }
+// 4th-kind Chebyshev sweep k, l1-Jacobi scaled (pressure_multigrid.glslinc:192-196):
+// the new direction from the previous one and the residual r = b - A x; the
+// sweep then writes x + d
+__device__ __forceinline__ float cheb_dir(int k, float dprev, float dinv, float r) {
+  return (k == 1 ? 0 : (float)(2 * k - 3) / (float)(2 * k + 1) * dprev) + (float)(8 * k - 4) / (float)(2 * k + 1) * dinv * r;
+}
+// Sweep 1 of a level whose x was just set to +0 by the kernel that also sets
+// its b (the V-cycle's load, the PCG update, a restriction): A x is then +0 in
+// every row (finite coefficients times +0, summed from +0; an inactive row
+// takes x itself), so r = b - A x = b and the sweep needs no neighbour.  The
+// producing kernel writes the sweep's d and x' = x + d = +0 + d itself, the
+// same values as the sweep's launch.
+__device__ __forceinline__ void cheb_first(float* d_out, float* x_out, float dinv, float b) {
+  const float d = cheb_dir(1, 0.0f, dinv, b - 0.0f);
+  *d_out = d; *x_out = 0.0f + d;
+}
 __device__ __forceinline__ bool mg_enabled(const Ctx& s, int phase) {
   const float* c = s.c;
   if (c[7] > .5f) return false;

@@ -7,6 +7,7 @@
 // shuffles so that their float sums match the reference's workgroup trees.
 #include "wv_motion.h"
 #include "wv_kernels.h"
+#include "wv_mgdev.h"
 
 #define GID() ((int64_t)blockIdx.x * blockDim.x + threadIdx.x)
 
@@ -759,7 +760,10 @@ __global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, int final) {
 // pressure_iterations.glslinc:23-30 (op 8) fused with the V-cycle's load of r
 // (pressure_multigrid.glslinc:191).  z = r/diag and the op-8 partials are
 // superseded by the V-cycle output and are not written.
-__global__ void __launch_bounds__(256) k_pcg_update(Ctx s, float* mg_b, float* mg_x1, float* mg_x2) {
+// The V-cycle that follows (op 9) starts by loading b = r, x = 0 into level 0;
+// that load and, with `fuse`, the level's first Chebyshev sweep (cheb_first)
+// are done here.  The update's gate solving() is the V-cycle's mg_enabled().
+__global__ void __launch_bounds__(256) k_pcg_update(Ctx s, MGDev m, int fuse) {
   if (!solving(s)) return;
   int64_t i = GID(); int64_t N = s.N; if (i >= N) return;
   float alpha = s.c[11];
@@ -767,7 +771,10 @@ __global__ void __launch_bounds__(256) k_pcg_update(Ctx s, float* mg_b, float* m
   int cb = candidate(s);
   putv(s, cb, P_(s) + i, getv(s, cb, P_(s) + i) + alpha * s.work[2 * N + i]);
   s.work[i] = r;
-  mg_b[i] = r; mg_x1[i] = 0; mg_x2[i] = 0;
+  float* v = m.vec + m.lv[0].vec_off;
+  v[i] = r; v[N + i] = 0;
+  if (fuse) cheb_first(&v[7 * N + i], &v[2 * N + i], v[4 * N + i], r);
+  else v[2 * N + i] = 0;
 }
 // pressure_iterations.glslinc:32 (op 10)
 __global__ void __launch_bounds__(256) k_direction(Ctx s) {
@@ -1081,8 +1088,8 @@ void wv_launch_spmv_dot(const Ctx& s, hipStream_t st) {
   hipLaunchKernelGGL(k_spmv_dot, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s);
   hipLaunchKernelGGL(k_dot_alpha, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
-void wv_launch_pcg_update(const Ctx& s, float* b0, float* x1, float* x2, hipStream_t st) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, b0, x1, x2);
+void wv_launch_pcg_update(const Ctx& s, const MGDev& m, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, m, (int)wv_mg_first_sweep_fused(m));
 }
 // masked 7-point SpMV launched on its own for live timing (writes only Ap)
 __global__ void __launch_bounds__(256) k_spmv_timed(Ctx s) {

@@ -366,12 +366,7 @@ static int dispatch(wv_solver* s, int op) {
     wv_launch_pcg_init_partials(c, s->stream);
     return 0;
   }
-  if (op == 8) {
-    const MGLevel& L0 = s->mg.lv[0];
-    float* v = s->mg.vec + L0.vec_off;
-    wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, s->stream);
-    return 0;
-  }
+  if (op == 8) { wv_launch_pcg_update(c, s->mg, s->stream); return 0; }
   int rc = wv_launch_op(c, op, s->stream);
   if (rc != 0) return fail(s, WV_EINVAL, "unknown op " + std::to_string(op));
   return 0;
@@ -856,8 +851,6 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
     else wv_mg_cycle(c, m, phase, s->smoothing, load, st);
   };
   cycle(0, true);                              // z = M r, rho, convergence (op 4)
-  const MGLevel& L0 = m.lv[0];
-  float* v = m.vec + L0.vec_off;
   int restarts = 0;
   const char* trace = getenv("WAVES_SOLVE_TRACE");
   const bool tracing = trace && trace[0] == '1';
@@ -870,7 +863,7 @@ extern "C" int wv_pressure_solve(wv_solver* s, int mode, float tolerance, int ma
     for (int k = 0; k < n; k++) {
       if (mode >= 1) { wv_launch_op(c, 5, st); wv_solve_dot_alpha64(c, st); }   // A p, alpha in FP64
       else wv_launch_spmv_dot(c, st);          // A p with the p.Ap partials, alpha
-      wv_launch_pcg_update(c, v, v + L0.count, v + 2 * L0.count, st);
+      wv_launch_pcg_update(c, m, st);
       cycle(1, false);                         // z = M r, beta, convergence (op 9)
       wv_launch_op(c, 10, st);                 // p = z + beta p
     }
