@@ -39,6 +39,7 @@ struct MGDev {
   float* previous;           // 9 N: previous trial's operator audit copy
   unsigned* state;           // hierarchy audit header + per-trial records
   unsigned* audit_part;      // audit counts per block stripe (AUDIT_STRIPES x 16), summed into state[1..7, 15, 16]
+  int16_t* ttab;             // the tail's index tables (wv_mg_tail_tables), null when it has none
   int lag, mode, capacity;
 };
 
@@ -60,6 +61,10 @@ void wv_launch_spmv_dot(const Ctx& s, hipStream_t st);  // ops 5 + 6 + 7 (plain 
 void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st);
 // first level of the one-workgroup tail for this hierarchy on the current device (-1: none)
 int wv_mg_tail_level(const MGDev& m);
+// 16-bit entries of the tail's index tables (0: the tail has no table form),
+// and their one-time fill on the device
+int64_t wv_mg_tail_table_shorts(const MGDev& m);
+void wv_mg_tail_tables(const MGDev& m, int16_t* tab, hipStream_t st);
 void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool load, hipStream_t st);
 void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st);
 void wv_launch_spmv_timed(const Ctx& s, hipStream_t st);

@@ -1237,6 +1237,267 @@ static int tail_field(int smoothing) {
   return smoothing % 2 ? 3 - down : down;
 }
 
+// ---- the tail with index tables ---------------------------------------------
+// When every transfer inside the tail aggregates (piecewise-constant, the
+// planner's levels >= 2) and its levels have fewer than 32768 nodes, the
+// neighbour index each row entry reads (the wrapped or clamped index of
+// k_mg_tail's apply), each coarse node's children and each fine node's parent
+// depend on the grid only.  They are tabulated once per solver as 16-bit
+// indices (k_mg_tail_tables) and loaded into LDS with the rows, so a phase is
+// loads and the row sums in stencil order, with no index arithmetic.  Only
+// the smoothed levels' rows are loaded (the last level is solved with L^-1),
+// all loads are 16 bytes wide, and the coarse solve's products are taken by
+// the whole workgroup before each row's serial sum.  Operations, operands and
+// orders are those of k_mg_tail.
+struct TailFast {
+  int T, L, words, tab, tab_words, linv, rhs, diag;
+  int coef[WV_MAXL], vec[WV_MAXL];            // word offsets in LDS
+  int J[WV_MAXL], ch[WV_MAXL], par[WV_MAXL];  // 16-bit entry offsets in the table region
+  int CH[WV_MAXL];                            // children per node of level l (restriction into l)
+};
+static bool tail_fast_layout(const MGDev& m, int T, TailFast& t) {
+  if (T < 1 || m.L - T < 2 || m.lv[m.L - 1].count > 64) return false;
+  for (int l = T; l < m.L; l++) if (m.lv[l].count >= 32768) return false;
+  for (int l = T; l < m.L - 1; l++) if (!m.lv[l].aggregate) return false;
+  t = TailFast{};
+  t.T = T; t.L = m.L;
+  auto up4 = [](int x) { return (x + 3) & ~3; };
+  int at = 0, sh = 0;
+  for (int l = T; l < m.L - 1; l++) { t.coef[l] = at; at = up4(at + m.lv[l].width * m.lv[l].count); }
+  for (int l = T; l < m.L; l++) { t.vec[l] = at; at = up4(at + 8 * m.lv[l].count); }
+  for (int l = T; l < m.L - 1; l++) { t.J[l] = sh; sh += m.lv[l].width * m.lv[l].count; }
+  for (int l = T + 1; l < m.L; l++) {
+    const MGLevel& P = m.lv[l - 1];
+    t.CH[l] = P.f[0] * P.f[1] * P.f[2]; t.ch[l] = sh; sh += t.CH[l] * m.lv[l].count;
+  }
+  for (int l = T; l < m.L - 1; l++) { t.par[l] = sh; sh += m.lv[l].count; }
+  sh = (sh + 7) & ~7;                  // whole 16-byte quads
+  t.tab = at; t.tab_words = sh / 2; at = up4(at + t.tab_words);
+  const int N = m.lv[m.L - 1].count;
+  t.linv = at; at = up4(at + N * N);
+  t.rhs = at; at += 64; t.diag = at; at += 64;
+  t.words = at;
+  return (size_t)t.words * sizeof(float) <= (size_t)tail_lds_limit();
+}
+// the table region's 16-bit entries (0 when the tail has no table form)
+int64_t wv_mg_tail_table_shorts(const MGDev& m) {
+  TailFast t;
+  return m.tail >= 1 && tail_fast_layout(m, m.tail, t) ? 2 * (int64_t)t.tab_words : 0;
+}
+__global__ void __launch_bounds__(256) k_mg_tail_tables(MGDev m, TailFast t, int16_t* tab) {
+  const int total = 2 * t.tab_words;
+  for (int e = (int)GID(); e < total; e += gridDim.x * blockDim.x) {
+    int v = -1;
+    for (int l = t.T; l < t.L - 1; l++) {          // row entries: k-major, the index apply reads
+      const MGLevel& L = m.lv[l]; const int n = L.count;
+      if (e < t.J[l] || e >= t.J[l] + L.width * n) continue;
+      const int k = (e - t.J[l]) / n, i = (e - t.J[l]) - k * n;
+      const I3 q = coord(i, mdims(m, l));
+      const int* o = m.offsets + 3 * (L.off_start + k);
+      if (L.wrap1) {
+        int j = i + m.deltas[L.off_start + k], zz = q.z + o[2];
+        j += zz < 0 ? n : (zz >= L.n[2] ? -n : 0);
+        if (m.perx) { int xx = q.x + o[0]; j += xx < 0 ? L.n[0] : (xx >= L.n[0] ? -L.n[0] : 0); }
+        v = min(max(j, 0), n - 1);
+      } else {
+        const int j = mwrap(m, l, mk(q.x + o[0], q.y + o[1], q.z + o[2]));
+        v = j >= 0 ? j : 0;
+      }
+    }
+    for (int l = t.T + 1; l < t.L; l++) {          // children of a coarse node: child-major, in child order
+      const int n = m.lv[l].count;
+      if (e < t.ch[l] || e >= t.ch[l] + t.CH[l] * n) continue;
+      const MGLevel& P = m.lv[l - 1];
+      const int c = (e - t.ch[l]) / n, I = (e - t.ch[l]) - c * n;
+      const I3 qc = coord(I, mdims(m, l));
+      const int x = c % P.f[0], y = (c / P.f[0]) % P.f[1], z = c / (P.f[0] * P.f[1]);
+      v = mwrap(m, l - 1, mk(qc.x * P.f[0] + x, qc.y * P.f[1] + y, qc.z * P.f[2] + z));
+    }
+    for (int l = t.T; l < t.L - 1; l++) {          // parent of a fine node
+      const MGLevel& L = m.lv[l];
+      if (e < t.par[l] || e >= t.par[l] + L.count) continue;
+      const I3 q = coord(e - t.par[l], mdims(m, l));
+      v = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), mdims(m, l + 1));
+    }
+    tab[e] = (int16_t)v;
+  }
+}
+void wv_mg_tail_tables(const MGDev& m, int16_t* tab, hipStream_t st) {
+  TailFast t;
+  if (!tab || m.tail < 1 || !tail_fast_layout(m, m.tail, t)) return;
+  hipLaunchKernelGGL(k_mg_tail_tables, dim3(64), dim3(256), 0, st, m, t, tab);
+}
+// n words from global to LDS, 16 bytes a load where the source allows, every
+// load of a thread's round issued before its stores
+__device__ __forceinline__ void tail_load_words(float* dst, const float* src, int n) {
+  if (((uintptr_t)src & 15) == 0) {
+    const int n4 = n >> 2; const float4* s4 = reinterpret_cast<const float4*>(src); float4* d4 = reinterpret_cast<float4*>(dst);
+    constexpr int U = 4;
+    for (int base = threadIdx.x; base < n4; base += U * TAIL_THREADS) {
+      float4 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { const int e = base + u * TAIL_THREADS; v[u] = s4[e < n4 ? e : n4 - 1]; }
+#pragma unroll
+      for (int u = 0; u < U; u++) { const int e = base + u * TAIL_THREADS; if (e < n4) d4[e] = v[u]; }
+    }
+    for (int e = 4 * n4 + (int)threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = src[e];
+  } else {
+    tail_copy(dst, n, [&](int e) { return src[e]; });
+  }
+}
+__device__ __forceinline__ void tail_load_bf16(float* dst, const uint16_t* src, int n) {
+  if (((uintptr_t)src & 7) == 0) {
+    const int n4 = n >> 2; const uint2* s4 = reinterpret_cast<const uint2*>(src); float4* d4 = reinterpret_cast<float4*>(dst);
+    constexpr int U = 4;
+    for (int base = threadIdx.x; base < n4; base += U * TAIL_THREADS) {
+      uint2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { const int e = base + u * TAIL_THREADS; v[u] = s4[e < n4 ? e : n4 - 1]; }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = base + u * TAIL_THREADS;
+        if (e < n4) d4[e] = make_float4(__uint_as_float(v[u].x << 16), __uint_as_float(v[u].x & 0xFFFF0000u),
+                                        __uint_as_float(v[u].y << 16), __uint_as_float(v[u].y & 0xFFFF0000u));
+      }
+    }
+    for (int e = 4 * n4 + (int)threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = bf16_to_f32(src[e]);
+  } else {
+    tail_copy(dst, n, [&](int e) { return bf16_to_f32(src[e]); });
+  }
+}
+__device__ __forceinline__ void tail_store_words(float* dst, const float* src, int n) {
+  if (((uintptr_t)dst & 15) == 0) {
+    const int n4 = n >> 2; const float4* s4 = reinterpret_cast<const float4*>(src); float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int e = threadIdx.x; e < n4; e += TAIL_THREADS) d4[e] = s4[e];
+    for (int e = 4 * n4 + (int)threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = src[e];
+  } else {
+    for (int e = threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = src[e];
+  }
+}
+template <bool H, int WC>
+__global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail_fast(Ctx s, MGDev m, TailFast tl, int phase, int smoothing) {
+  if (!mg_enabled(s, phase)) return;
+  extern __shared__ float sm[];
+  const int tid = threadIdx.x;
+  for (int l = tl.T; l < tl.L - 1; l++) {
+    const MGLevel& L = m.lv[l];
+    if (H) tail_load_bf16(sm + tl.coef[l], m.rows16 + L.row_off, L.width * L.count);
+    else tail_load_words(sm + tl.coef[l], m.rows + L.row_off, L.width * L.count);
+  }
+  for (int l = tl.T; l < tl.L; l++) tail_load_words(sm + tl.vec[l], m.vec + m.lv[l].vec_off, 8 * m.lv[l].count);
+  tail_load_words(sm + tl.tab, reinterpret_cast<const float*>(m.ttab), tl.tab_words);
+  const int lc = tl.L - 1, N = m.lv[lc].count;
+  tail_load_words(sm + tl.linv, m.linv, N * N);
+  __syncthreads();
+  const int16_t* tab = reinterpret_cast<const int16_t*>(sm + tl.tab);
+  auto vec = [&](int l, int field) { return sm + tl.vec[l] + field * m.lv[l].count; };
+  auto active = [&](int l, int i) { return vec(l, 3)[i] > .5f; };
+  // sum_k A[i, k] x[J[i, k]] in stencil order
+  auto apply = [&](int l, int source, int i) {
+    const int n = m.lv[l].count;
+    const float* A = sm + tl.coef[l]; const int16_t* J = tab + tl.J[l]; const float* x = vec(l, source);
+    float sum = 0;
+    if (WC > 0) {
+      float a[WC > 0 ? WC : 1]; int j[WC > 0 ? WC : 1];
+#pragma unroll
+      for (int k = 0; k < WC; k++) { a[k] = A[k * n + i]; j[k] = J[k * n + i]; }
+#pragma unroll
+      for (int k = 0; k < WC; k++) a[k] = a[k] * x[j[k]];
+#pragma unroll
+      for (int k = 0; k < WC; k++) sum += a[k];
+    } else {
+      const int W = m.lv[l].width;
+#pragma unroll 9
+      for (int k = 0; k < W; k++) sum += A[k * n + i] * x[J[k * n + i]];
+    }
+    return sum;
+  };
+  auto sweep = [&](int l, int source, int k) {
+    const int n = m.lv[l].count;
+    for (int i = tid; i < n; i += TAIL_THREADS) {
+      const float xs = vec(l, source)[i], b = vec(l, 0)[i], dprev = vec(l, 7)[i], dinv = vec(l, 4)[i];
+      const bool on = active(l, i);
+      float ax = apply(l, source, i);
+      if (!on) ax = xs;
+      const float d = cheb_dir(k, dprev, dinv, b - ax);
+      vec(l, 7)[i] = d; vec(l, 3 - source)[i] = xs + d;
+    }
+    __syncthreads();
+  };
+  auto residual = [&](int l, int field) {
+    const int n = m.lv[l].count;
+    for (int i = tid; i < n; i += TAIL_THREADS) {
+      const float xs = vec(l, field)[i], b = vec(l, 0)[i];
+      const bool on = active(l, i);
+      float ax = apply(l, field, i);
+      if (!on) ax = xs;
+      vec(l, 6)[i] = b - ax;
+    }
+    __syncthreads();
+  };
+  // aggregation into level c: the active children's residuals in child order
+  auto restrict_to = [&](int c) {
+    const int n = m.lv[c].count, CH = tl.CH[c]; const int16_t* ch = tab + tl.ch[c];
+    for (int I = tid; I < n; I += TAIL_THREADS) {
+      float sum = 0;
+      if (active(c, I)) {
+        for (int t = 0; t < CH; t++) {
+          const int j = ch[t * n + I];
+          if (j < 0 || !active(c - 1, j)) continue;
+          sum += 1.0f * vec(c - 1, 6)[j];
+        }
+      }
+      vec(c, 0)[I] = sum; vec(c, 1)[I] = 0;
+      if (c < tl.L - 1) cheb_first(&vec(c, 7)[I], &vec(c, 2)[I], vec(c, 4)[I], sum);
+      else vec(c, 2)[I] = 0;
+    }
+    __syncthreads();
+  };
+  auto prolong = [&](int l, int ff, int cf) {
+    const int n = m.lv[l].count; const int16_t* par = tab + tl.par[l];
+    for (int i = tid; i < n; i += TAIL_THREADS) {
+      float add = 0;
+      if (active(l, i)) { const int j = par[i]; if (active(l + 1, j)) add += 1 * vec(l + 1, cf)[j]; }
+      vec(l, ff)[i] += add;
+    }
+    __syncthreads();
+  };
+  int fields[WV_MAXL];
+  for (int l = tl.T; l < tl.L - 1; l++) {
+    int f = 2;   // sweep 1 done where b was set (cheb_first)
+    for (int k = 2; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
+    fields[l] = f;
+    residual(l, f);
+    restrict_to(l + 1);
+  }
+  {  // x = L^-T (L^-1 b) on the last level: the products first, then each row's sum in order
+    float* Li = sm + tl.linv; float* rhs = sm + tl.rhs; float* dg = sm + tl.diag; const float* b = vec(lc, 0);
+    for (int e = tid; e < N * N; e += TAIL_THREADS) {
+      const int t = e / N, j = e - t * N;   // L^-1 row t, column j: the upper triangle takes the products
+      if (j < t) Li[j * N + t] = Li[t * N + j] * b[j];
+      else if (j == t) dg[t] = Li[t * N + t] * b[t];
+    }
+    __syncthreads();
+    if (tid < N) { float d = 0; for (int j = 0; j < tid; j++) d += Li[j * N + tid]; d += dg[tid]; rhs[tid] = d; }
+    __syncthreads();
+    for (int e = tid; e < N * N; e += TAIL_THREADS) {
+      const int r = e / N, t = e - r * N;   // entry (r, t) with r >= t: L^-1[r, t] rhs[r], in place
+      if (r >= t) Li[r * N + t] = Li[r * N + t] * rhs[r];
+    }
+    __syncthreads();
+    if (tid < N) { float d = 0; for (int j = tid; j < N; j++) d += Li[j * N + tid]; vec(lc, 1)[tid] = d; }
+    __syncthreads();
+  }
+  int cf = 1;
+  for (int l = tl.L - 2; l >= tl.T; l--) {
+    int f = fields[l];
+    prolong(l, f, cf);
+    for (int k = 1; k <= smoothing; k++) { sweep(l, f, k); f = 3 - f; }
+    cf = f;
+  }
+  for (int l = tl.T; l < tl.L; l++) tail_store_words(m.vec + m.lv[l].vec_off, sm + tl.vec[l], 8 * m.lv[l].count);
+}
+
 // dynamic LDS of k_mg_coarse: the rows' product slices when G > 1 lanes share a row
 static size_t coarse_lds(const MGDev& m, int l) {
   const MGLevel& L = m.lv[l];
@@ -1269,6 +1530,14 @@ struct CycleBuilder {
   // water/pressure_multigrid.gd:141-154
   int cycle(int l) {
     if (l > 0 && l == m.tail) {
+      TailFast tf;
+      if (m.ttab && tail_fast_layout(m, l, tf)) {
+        bool w27 = true;
+        for (int k = l; k < m.L - 1; k++) w27 = w27 && m.lv[k].width == 27;
+        if (w27) hipLaunchKernelGGL((k_mg_tail_fast<H, 27>), dim3(1), dim3(TAIL_THREADS), tf.words * sizeof(float), st, s, m, tf, phase, smoothing);
+        else hipLaunchKernelGGL((k_mg_tail_fast<H, 0>), dim3(1), dim3(TAIL_THREADS), tf.words * sizeof(float), st, s, m, tf, phase, smoothing);
+        return tail_field(smoothing);
+      }
       TailLayout tl = tail_layout(m, l);
       hipLaunchKernelGGL(k_mg_tail<H>, dim3(1), dim3(TAIL_THREADS), tl.floats * sizeof(float), st, s, m, tl, phase, smoothing);
       return tail_field(smoothing);

@@ -315,6 +315,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(m.previous, 9 * c.N);
   ALLOC(m.state, 64 + 32 * s->history_capacity);
   ALLOC(m.audit_part, 64 * 16);
+  { const int64_t ts = wv_mg_tail_table_shorts(m); if (ts > 0) ALLOC(m.ttab, ts); }
 #undef ALLOC
   m.offsets = offs; m.slots = slots; m.deltas = deltas;
   m.lag = (int)o[2]; m.mode = (int)o[4]; m.capacity = (int)s->history_capacity;
@@ -332,6 +333,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
     c.tick_ns = (float)(1e6 / khz);
   }
   wv_launch_reflect_table(c, s->stream);
+  wv_mg_tail_tables(m, m.ttab, s->stream);
   if (hipStreamSynchronize(s->stream) != hipSuccess) { release(s); delete s; return WV_EHIP; }
   *out = s;
   return WV_OK;
