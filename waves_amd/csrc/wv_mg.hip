@@ -1379,6 +1379,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail_fast(Ctx s, MGDev m, T
   if (!mg_enabled(s, phase)) return;
   extern __shared__ float sm[];
   const int tid = threadIdx.x;
+  const int lc = tl.L - 1, N = m.lv[lc].count;
   for (int l = tl.T; l < tl.L - 1; l++) {
     const MGLevel& L = m.lv[l];
     if (H) tail_load_bf16(sm + tl.coef[l], m.rows16 + L.row_off, L.width * L.count);
@@ -1386,7 +1387,6 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail_fast(Ctx s, MGDev m, T
   }
   for (int l = tl.T; l < tl.L; l++) tail_load_words(sm + tl.vec[l], m.vec + m.lv[l].vec_off, 8 * m.lv[l].count);
   tail_load_words(sm + tl.tab, reinterpret_cast<const float*>(m.ttab), tl.tab_words);
-  const int lc = tl.L - 1, N = m.lv[lc].count;
   tail_load_words(sm + tl.linv, m.linv, N * N);
   __syncthreads();
   const int16_t* tab = reinterpret_cast<const int16_t*>(sm + tl.tab);
@@ -1478,14 +1478,35 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail_fast(Ctx s, MGDev m, T
       else if (j == t) dg[t] = Li[t * N + t] * b[t];
     }
     __syncthreads();
-    if (tid < N) { float d = 0; for (int j = 0; j < tid; j++) d += Li[j * N + tid]; d += dg[tid]; rhs[tid] = d; }
+    // each row's products added in order, eight loads ahead of their additions
+    if (tid < N) {
+      float d = 0;
+      for (int j0 = 0; j0 < tid; j0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = Li[(j0 + u < tid ? j0 + u : 0) * N + tid];
+#pragma unroll
+        for (int u = 0; u < 8; u++) if (j0 + u < tid) d += v[u];
+      }
+      d += dg[tid]; rhs[tid] = d;
+    }
     __syncthreads();
     for (int e = tid; e < N * N; e += TAIL_THREADS) {
       const int r = e / N, t = e - r * N;   // entry (r, t) with r >= t: L^-1[r, t] rhs[r], in place
       if (r >= t) Li[r * N + t] = Li[r * N + t] * rhs[r];
     }
     __syncthreads();
-    if (tid < N) { float d = 0; for (int j = tid; j < N; j++) d += Li[j * N + tid]; vec(lc, 1)[tid] = d; }
+    if (tid < N) {
+      float d = 0;
+      for (int j0 = tid; j0 < N; j0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = Li[(j0 + u < N ? j0 + u : tid) * N + tid];
+#pragma unroll
+        for (int u = 0; u < 8; u++) if (j0 + u < N) d += v[u];
+      }
+      vec(lc, 1)[tid] = d;
+    }
     __syncthreads();
   }
   int cf = 1;
