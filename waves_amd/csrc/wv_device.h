@@ -48,7 +48,8 @@ struct Ctx {
   float* aux;          // 2 N scratch of the learned warm start (A v1, A v2)
   int flex;            // flexible (Polak-Ribiere) CG coefficient in the FP64 V-cycle output
   int* mq;             // faces whose MacCormack trace needs the general sampling path (NF)
-  unsigned* mq_count;  // queue lengths: [0] forward trace (op 27), [1] correction (op 28)
+  unsigned* mq_count;  // queue lengths: [0] forward trace (op 27), [1] correction (op 28), [2] exact
+                       // distance; then 8 sub-queue counters per queue slot (4 slots)
   signed char* reflect;  // mirror code of the wall search per (component, y, x), 3 (ny + 1) (nx + 1)
   const float* quota;  // device copy of the request quota (steps >= 0, -deadline < 0)
   // pressure-section timing on the device clock (telemetry, no reference counterpart)

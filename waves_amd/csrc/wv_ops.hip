@@ -80,19 +80,52 @@ __global__ void __launch_bounds__(256) k_perturb(Ctx s) {
 
 // ------------------------------------------------------- geometry (17,18,16) --
 // Work queues (Ctx::mq): a first pass handles the cells or faces on a cheap
-// path and appends the others, one atomic per wave; a second pass works the
-// queue in dense waves.  Slots: 0/1 MacCormack ops 27/28, 2 distance.
-__device__ __forceinline__ void queue_append(const Ctx& s, bool defer, int64_t i, int slot) {
-  unsigned long long ballot = __ballot(defer);
-  if (!ballot) return;
-  int lane = __lane_id(), leader = __ffsll((long long)ballot) - 1;
-  int base = 0;
-  if (lane == leader) base = (int)atomicAdd(&s.mq_count[slot], (unsigned)__popcll(ballot));
-  base = __shfl(base, leader);
-  if (defer) s.mq[base + __popcll(ballot & ((1ull << lane) - 1))] = (int)i;
+// path and appends the others; a second pass works the queue in dense waves.
+// Slots: 0/1 MacCormack ops 27/28, 2 distance.
+__global__ void k_queue_reset(Ctx s, int slot) {
+  if (blockIdx.x == 0 && threadIdx.x < 9) { if (threadIdx.x == 8) s.mq_count[slot] = 0; else s.mq_count[4 + 8 * slot + threadIdx.x] = 0; }
 }
-__global__ void k_queue_reset(Ctx s, int slot) { if (threadIdx.x == 0 && blockIdx.x == 0) s.mq_count[slot] = 0; }
+// Block-aggregated work queues: a launch's items go to eight sub-queues, one
+// per XCD of the block order (block b to sub-queue b mod 8, as MI355X hands
+// workgroups to its eight XCDs), each with its own counter, and each block
+// takes its slots with one atomic.  Thousands of single-address atomics from
+// the whole chip no longer serialise the pass.  Sub-queue k holds the items of
+// blocks b = k mod 8 at mq[k * cap ...] with cap = ceil(blocks / 8) * block
+// size, which bounds its length (mq has room for 8 cap items: it is sized for
+// the face count plus 8 blocks).  Every thread of the block makes the call.
+__device__ __forceinline__ void queue_append_block(const Ctx& s, bool defer, int64_t i, int slot, int cap) {
+  __shared__ unsigned wcount[16], wbase[16], bbase;
+  const int lane = __lane_id(), wave = (int)(threadIdx.x >> 6), nw = (int)((blockDim.x + 63) >> 6);
+  const unsigned long long ballot = __ballot(defer);
+  if (lane == 0) wcount[wave] = (unsigned)__popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+    for (int w = 0; w < nw; w++) { wbase[w] = tot; tot += wcount[w]; }
+    bbase = tot ? atomicAdd(&s.mq_count[4 + 8 * slot + (blockIdx.x & 7)], tot) : 0u;
+  }
+  __syncthreads();
+  if (defer) s.mq[(int64_t)(blockIdx.x & 7) * cap + bbase + wbase[wave] + (unsigned)__popcll(ballot & ((1ull << lane) - 1))] = (int)i;
+}
+// the sub-queues' lengths (cnt) and their sum; item v of the concatenation
+struct QueueView {
+  unsigned cnt[8]; int64_t total; int cap;
+  __device__ __forceinline__ int item(const Ctx& s, int64_t v) const {
+    int k = 0; int64_t base = 0;
+    while (k < 7 && v >= base + cnt[k]) { base += cnt[k]; k++; }
+    return s.mq[(int64_t)k * cap + (v - base)];
+  }
+};
+__device__ __forceinline__ QueueView queue_view(const Ctx& s, int slot, int cap) {
+  QueueView q; q.total = 0; q.cap = cap;
+#pragma unroll
+  for (int k = 0; k < 8; k++) { q.cnt[k] = s.mq_count[4 + 8 * slot + k]; q.total += q.cnt[k]; }
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.mq_count[slot] = (unsigned)q.total;   // telemetry
+  return q;
+}
 constexpr int QUEUE_BLOCKS = 4096;
+// sub-queue capacity of a launch of `items` threads in blocks of `block`
+static inline int queue_cap(int64_t items, int block) { return (int)(((items + block - 1) / block + 7) / 8) * block; }
 // MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
 template <int MODE>
 __device__ __forceinline__ bool reconstruct_gate(const Ctx& s) {
@@ -124,7 +157,7 @@ __global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
 __device__ __forceinline__ void write_distance(const Ctx& s, int64_t i, I3 q, float ph) {
   s.geo[i] = make_float2(ph, solid(s, q) ? -1.0f : (ph < 0 ? 1.0f : 0.0f));
 }
-__global__ void __launch_bounds__(256) k_distance_direct(Ctx s) {
+__global__ void __launch_bounds__(256) k_distance_direct(Ctx s, int cap) {
   if (s.c[4] > .5f && s.c[5] < .5f) return;
   int64_t i = GID(); bool defer = false;
   if (i < s.N) {
@@ -132,20 +165,21 @@ __global__ void __launch_bounds__(256) k_distance_direct(Ctx s) {
     float ph = distance_head(s, q, geometry_bank(s, false), seed);
     if (seed) defer = true; else write_distance(s, i, q, ph);
   }
-  queue_append(s, defer, i, 2);
+  queue_append_block(s, defer, i, 2, cap);
 }
 // lane k < 27 of a 32-lane group evaluates neighbour k; lane 0 gathers the 27
 // (box, candidate) pairs and runs the reference's pruned search in order
 constexpr int DIST_THREADS = 64;  // threads per block of the queued distance pass
-__global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s) {
+__global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s, int cap) {
   if (s.c[4] > .5f && s.c[5] < .5f) return;
   __shared__ float verts[2 * VERT_FLOATS * DIST_THREADS];
   const Verts<DIST_THREADS> va{verts + threadIdx.x}, vb{verts + VERT_FLOATS * DIST_THREADS + threadIdx.x};
-  const int64_t n = s.mq_count[2], groups = (int64_t)gridDim.x * blockDim.x / 32;
+  const QueueView qv = queue_view(s, 2, cap);
+  const int64_t n = qv.total, groups = (int64_t)gridDim.x * blockDim.x / 32;
   const int sub = (int)(threadIdx.x & 31);
   const int gb = geometry_bank(s, false);
   for (int64_t j = GID() / 32; j < n; j += groups) {
-    int64_t i = s.mq[j]; I3 q = coord(i, dims(s));
+    int64_t i = qv.item(s, j); I3 q = coord(i, dims(s));
     float box = __int_as_float(0x7f800000), m = box;
     if (sub < 27) distance_neighbour(s, q, sub, gb, box, m, va, vb);
     bool seed; float head = distance_head(s, q, gb, seed);
@@ -482,20 +516,21 @@ __global__ void __launch_bounds__(256) k_maccormack(Ctx s) {
 // the others (one atomic per wave); the second traces the queued faces in
 // dense waves.  Each face is computed by the same code either way.
 template <bool CORRECT>
-__global__ void __launch_bounds__(256) k_maccormack_direct(Ctx s) {
+__global__ void __launch_bounds__(256) k_maccormack_direct(Ctx s, int cap) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
   int64_t i = GID();
   bool defer = i < s.NF && !maccormack_face<CORRECT, true, true>(s, i);
-  queue_append(s, defer, i, CORRECT ? 1 : 0);
+  queue_append_block(s, defer, i, CORRECT ? 1 : 0, cap);
 }
 // eight lanes per queued face: the deferred traces are few and latency-bound,
 // and the corners of a general sample are independent donor evaluations
 template <bool CORRECT>
-__global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s) {
+__global__ void __launch_bounds__(256) k_maccormack_deferred(Ctx s, int cap) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  const int64_t n = s.mq_count[CORRECT ? 1 : 0], groups = (int64_t)gridDim.x * blockDim.x / 8;
+  const QueueView qv = queue_view(s, CORRECT ? 1 : 0, cap);
+  const int64_t n = qv.total, groups = (int64_t)gridDim.x * blockDim.x / 8;
   const int sub = (int)(threadIdx.x & 7);
-  for (int64_t j = GID() / 8; j < n; j += groups) maccormack_face<CORRECT, true, false, true>(s, s.mq[j], sub);
+  for (int64_t j = GID() / 8; j < n; j += groups) maccormack_face<CORRECT, true, false, true>(s, qv.item(s, j), sub);
 }
 
 // The wall-search code of every in-range face at z = 0, per (component, y, x)
@@ -1032,8 +1067,8 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 17: hipLaunchKernelGGL(k_reconstruct<0>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 18:
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
-    hipLaunchKernelGGL(k_distance_direct, dim3(blocks(s.N, 64)), dim3(64), 0, st, s);  // one wave per block
-    hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s);
+    hipLaunchKernelGGL(k_distance_direct, dim3(blocks(s.N, 64)), dim3(64), 0, st, s, queue_cap(s.N, 64));  // one wave per block
+    hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s, queue_cap(s.N, 64));
     break;
   case 19: hipLaunchKernelGGL(k_reconstruct<2>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
   case 20: LRED(k_cfl, 3); break;
@@ -1046,10 +1081,11 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
     if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, corr ? 1 : 0);
     // one wave per block: waves whose faces take long paths hold no other wave's slot
-    if (corr) hipLaunchKernelGGL(k_maccormack_direct<true>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
-    else hipLaunchKernelGGL(k_maccormack_direct<false>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s);
-    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s);  // one wave per block
-    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s);
+    const int cap = queue_cap(s.NF, 64);
+    if (corr) hipLaunchKernelGGL(k_maccormack_direct<true>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s, cap);
+    else hipLaunchKernelGGL(k_maccormack_direct<false>, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s, cap);
+    if (corr) hipLaunchKernelGGL(k_maccormack_deferred<true>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s, cap);  // one wave per block
+    else hipLaunchKernelGGL(k_maccormack_deferred<false>, dim3(QUEUE_BLOCKS * 4), dim3(64), 0, st, s, cap);
     break;
   }
   case 30: L256(k_ext_seed, s.NF); break;

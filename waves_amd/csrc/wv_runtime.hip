@@ -286,8 +286,8 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.dctl, 16);
   ALLOC(c.pacc, c.N);
   ALLOC(c.aux, 2 * c.N);
-  ALLOC(c.mq, c.NF);
-  ALLOC(c.mq_count, 4);
+  ALLOC(c.mq, c.NF + 8 * 256);   // the face count plus eight blocks (block-aggregated sub-queues)
+  ALLOC(c.mq_count, 4 + 4 * 8);  // per queue: its length, then eight sub-queue counters
   ALLOC(c.reflect, 3 * (int64_t)(c.nx + 1) * (c.ny + 1));
   ALLOC(c.tstate, 8);
   ALLOC(c.step_times, 4 * (s->history_capacity + 1));
