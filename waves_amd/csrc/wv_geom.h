@@ -47,6 +47,42 @@ __device__ __forceinline__ float inverse_volume(F3 n, float fraction) {
   return alpha;
 }
 
+// inverse_volume on the eight lanes of a group (lanes 8g .. 8g + 7 of a wave,
+// sub = lane & 7), all holding the same n and fraction.  Each round evaluates
+// the midpoints of the next three bisection levels at once -- lane r takes
+// node r of the round's tree, its interval halved from the round's (lo, hi)
+// along r's path exactly as the serial loop would -- then every lane follows
+// the decisions down the tree.  The midpoints, the tests and the order of the
+// halvings are the serial loop's: the result is bit-identical, in 9 rounds of
+// dependent work instead of 25.
+__device__ __forceinline__ float inverse_volume8(F3 n, float fraction, int sub) {
+  float lo = gmin(n.x, 0.0f) + gmin(n.y, 0.0f) + gmin(n.z, 0.0f);
+  float hi = gmax(n.x, 0.0f) + gmax(n.y, 0.0f) + gmax(n.z, 0.0f);
+  for (int done = 0; done < 25;) {
+    const int levels = 25 - done >= 3 ? 3 : 25 - done;
+    int test = 0;
+    if (sub >= 1 && sub < (1 << levels)) {
+      float l = lo, h = hi;
+      const int depth = 31 - __clz(sub);
+      for (int b = depth - 1; b >= 0; b--) { const float m = 0.5f * (l + h); if ((sub >> b) & 1) l = m; else h = m; }
+      test = cube_volume(n, 0.5f * (l + h)) < fraction;
+    }
+    int node = 1;
+    for (int lev = 0; lev < levels; lev++) {
+      const int t = __shfl(test, node, 8);
+      const float mid = 0.5f * (lo + hi);
+      if (t) lo = mid; else hi = mid;
+      node = 2 * node + t;
+    }
+    done += levels;
+  }
+  float alpha = 0.5f * (lo + hi), ctr = dot3(n, mkf(.5f, .5f, .5f));
+  if (fraction == .5f) return ctr;
+  if (fraction < .5f) return gmin(alpha, ctr);
+  if (alpha <= ctr) alpha = ctr + 5.96046448e-8f * dot3(mkf(fabsf(n.x), fabsf(n.y), fabsf(n.z)), mkf(1, 1, 1));
+  return alpha;
+}
+
 // ---- coastal_target.glslinc ----
 __device__ __forceinline__ float smooth_ramp(float q) { q = gclamp(q, 0, 1); return q * q * q * (10 + q * (-15 + 6 * q)); }
 __device__ __forceinline__ float ramped_amplitude(const Ctx& s, float time) {
@@ -126,13 +162,9 @@ __device__ __forceinline__ float hf_fraction_known(const Ctx& s, I3 q, int sourc
 // flags and fractions are read first, 27 independent loads issued together,
 // and the loops below (fully unrolled, constant block indices) run on
 // registers.  The values and the order of the arithmetic are the reference's.
-__device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source, int gbank) {
-  bool sb[27]; float fb[27];
-#pragma unroll
-  for (int c = 0; c < 27; c++) {
-    I3 p = mk(q.x + c % 3 - 1, q.y + (c / 3) % 3 - 1, q.z + c / 9 - 1);
-    sb[c] = solid(s, p); fb[c] = hf_fraction_known(s, p, source, gbank, sb[c]);
-  }
+// the gradient from the block's solid flags sb and fractions fb (cell c at
+// offset (c % 3 - 1, c / 3 % 3 - 1, c / 9 - 1))
+__device__ __forceinline__ F3 youngs_from_block(const Ctx& s, const bool* sb, const float* fb) {
   F3 g = mkf(0, 0, 0);
 #pragma unroll
   for (int axis = 0; axis < 3; axis++) {
@@ -160,6 +192,31 @@ __device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source,
     setc(g, axis, acc);
   }
   return g;
+}
+__device__ __forceinline__ F3 interface_gradient(const Ctx& s, I3 q, int source, int gbank) {
+  bool sb[27]; float fb[27];
+#pragma unroll
+  for (int c = 0; c < 27; c++) {
+    I3 p = mk(q.x + c % 3 - 1, q.y + (c / 3) % 3 - 1, q.z + c / 9 - 1);
+    sb[c] = solid(s, p); fb[c] = hf_fraction_known(s, p, source, gbank, sb[c]);
+  }
+  return youngs_from_block(s, sb, fb);
+}
+// The same gradient on the eight lanes of a group (lanes 8g .. 8g + 7 of a
+// wave, sub = lane & 7, all for the same cell): lane sub evaluates the block's
+// cells sub, sub + 8, ..., which every lane then gathers by shuffles.
+__device__ __forceinline__ F3 interface_gradient8(const Ctx& s, I3 q, int source, int gbank, int sub) {
+  int msb[4]; float mfb[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int c = sub + 8 * u < 27 ? sub + 8 * u : 26;
+    const I3 p = mk(q.x + c % 3 - 1, q.y + (c / 3) % 3 - 1, q.z + c / 9 - 1);
+    const bool sc = solid(s, p); msb[u] = sc; mfb[u] = hf_fraction_known(s, p, source, gbank, sc);
+  }
+  bool sb[27]; float fb[27];
+#pragma unroll
+  for (int c = 0; c < 27; c++) { sb[c] = __shfl(msb[c >> 3], c & 7, 8) != 0; fb[c] = __shfl(mfb[c >> 3], c & 7, 8); }
+  return youngs_from_block(s, sb, fb);
 }
 __device__ __forceinline__ bool hf_valid(const Ctx& s, I3 q) { q = wrap_cell(s, q); return !outside(s, q) && !solid(s, q); }
 // height_normal.glslinc:6-34 (numerics flag bit 0)
@@ -203,6 +260,11 @@ __device__ __forceinline__ F3 normal_from_gradient(const Ctx& s, I3 q, F3 g, int
 __device__ __forceinline__ float4 plane_from_gradient(const Ctx& s, I3 q, F3 g, int source, int gbank, float f) {
   F3 n = normal_from_gradient(s, q, g, source, gbank);
   return make_float4(n.x, n.y, n.z, inverse_volume(n, f));
+}
+// the plane of an interface cell on the eight lanes of a group
+__device__ __forceinline__ float4 plane8(const Ctx& s, I3 q, int source, int gbank, float f, int sub) {
+  F3 n = normal_from_gradient(s, q, interface_gradient8(s, q, source, gbank, sub), source, gbank);
+  return make_float4(n.x, n.y, n.z, inverse_volume8(n, f, sub));
 }
 
 // pressure_geometry.glslinc:99-101

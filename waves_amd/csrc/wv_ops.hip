@@ -150,6 +150,43 @@ __global__ void __launch_bounds__(256) k_reconstruct(Ctx s) {
   reconstruct_store<MODE>(s, i, e);
   if (MODE == 2) { float fa = getv(s, accepted(s), i); s.img_kinds[i] = fa <= 0 ? 0 : (fa >= 1 ? 1 : 2); }
 }
+// The same planes in two passes: full and empty cells (almost all) directly,
+// interface cells appended to a work queue and reconstructed by eight lanes
+// each (interface_gradient8, inverse_volume8), with the same arithmetic.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_reconstruct_direct(Ctx s, int cap) {
+  if (!reconstruct_gate<MODE>(s)) return;
+  const int64_t i = GID(); bool defer = false;
+  if (i < s.N) {
+    const I3 q = coord(i, dims(s));
+    const int gbank = geometry_bank(s, MODE == 2);
+    const float f = MODE == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+    if (f <= 0 || f >= 1) reconstruct_store<MODE>(s, i, make_float4(0, 1, 0, f)); else defer = true;
+    if (MODE == 2) { float fa = getv(s, accepted(s), i); s.img_kinds[i] = fa <= 0 ? 0 : (fa >= 1 ? 1 : 2); }
+  }
+  queue_append_block(s, defer, i, 3, cap);
+}
+template <int MODE>
+__global__ void __launch_bounds__(256) k_reconstruct_deferred(Ctx s, int cap) {
+  if (!reconstruct_gate<MODE>(s)) return;
+  const QueueView qv = queue_view(s, 3, cap);
+  const int64_t groups = (int64_t)gridDim.x * blockDim.x / 8;
+  const int sub = (int)(threadIdx.x & 7);
+  const int gbank = geometry_bank(s, MODE == 2), source = MODE == 1 ? 1 : 0;
+  for (int64_t j = GID() / 8; j < qv.total; j += groups) {
+    const int64_t i = qv.item(s, j); const I3 q = coord(i, dims(s));
+    const float f = MODE == 1 ? transport_f(s, q) : geometry_fraction(s, q, gbank);
+    const float4 e = plane8(s, q, source, gbank, f, sub);
+    if (sub == 0) reconstruct_store<MODE>(s, i, e);
+  }
+}
+template <int MODE>
+static void launch_reconstruct(const Ctx& s, hipStream_t st) {
+  const int cap = queue_cap(s.N, 256);
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 3);
+  hipLaunchKernelGGL(k_reconstruct_direct<MODE>, dim3((unsigned)((s.N + 255) / 256)), dim3(256), 0, st, s, cap);
+  hipLaunchKernelGGL(k_reconstruct_deferred<MODE>, dim3(1024), dim3(256), 0, st, s, cap);
+}
 // pressure_pass.glslinc:131-137 (op 18)
 // The exact distance in two passes: interface cells and cells with no
 // opposite-side neighbour (almost all) directly; the seeded cells along the
@@ -1064,16 +1101,16 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 12: LRED2(k_accept); break;
   case 14: L256(k_export_plain, s.N); break;
   case 16: L256(k_coefficients, s.N); break;
-  case 17: hipLaunchKernelGGL(k_reconstruct<0>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
+  case 17: launch_reconstruct<0>(s, st); break;
   case 18:
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, s, 2);
     hipLaunchKernelGGL(k_distance_direct, dim3(blocks(s.N, 64)), dim3(64), 0, st, s, queue_cap(s.N, 64));  // one wave per block
     hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s, queue_cap(s.N, 64));
     break;
-  case 19: hipLaunchKernelGGL(k_reconstruct<2>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
+  case 19: launch_reconstruct<2>(s, st); break;
   case 20: LRED(k_cfl, 3); break;
   case 22: L256(k_bank_copy, s.stride); break;
-  case 23: hipLaunchKernelGGL(k_reconstruct<1>, dim3(blocks(s.N, 64)), dim3(64), 0, st, s); break;  // one wave per block: a wave of mixed cells holds no other wave's slot
+  case 23: launch_reconstruct<1>(s, st); break;
   case 24: L256(k_flux, s.NF); break;
   case 25: LRED2(k_vof_update); break;
   case 27: case 28: {
