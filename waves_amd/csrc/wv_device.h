@@ -47,6 +47,8 @@ struct Ctx {
   double* pacc;        // FP64 pressure accumulator of the mixed-precision solve (N)
   float* aux;          // 2 N scratch of the learned warm start (A v1, A v2)
   int flex;            // flexible (Polak-Ribiere) CG coefficient in the FP64 V-cycle output
+  float4* vpart;       // 4 G + 4, G = ceil(N / 128): the VOF update's group partials per sweep and the
+                       // sweep headers whose ledger sums the acceptance finaliser adds (batch program)
   int* mq;             // faces whose MacCormack trace needs the general sampling path (NF)
   unsigned* mq_count;  // queue lengths: [0] forward trace (op 27), [1] correction (op 28), [2] exact
                        // distance; then 8 sub-queue counters per queue slot (4 slots)

@@ -54,6 +54,11 @@ void wv_launch_step_record(const Ctx& s, hipStream_t st);
 // V-cycle's first sweep when wv_mg_first_sweep_fused)
 void wv_launch_pcg_update(const Ctx& s, const MGDev& m, hipStream_t st);
 bool wv_mg_first_sweep_fused(const MGDev& m);
+// batch program: ops 25 + 26 of VOF sweep `sweep` with the finaliser's two
+// in-order ledger sums left to op 12's finaliser, and op 12 (+ the sweeps'
+// ledger sums when `vof_ledgers`)
+void wv_launch_vof_deferred(const Ctx& s, int sweep, hipStream_t st);
+void wv_launch_accept(const Ctx& s, bool vof_ledgers, hipStream_t st);
 void wv_launch_spmv_dot(const Ctx& s, hipStream_t st);  // ops 5 + 6 + 7 (plain FP32 form)
 // Multigrid: build = hierarchy audit + Galerkin rebuild (phase 2);
 // cycle = one V-cycle (phase 0 initial / 1 iteration) ending in the fused

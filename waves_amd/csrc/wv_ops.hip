@@ -124,6 +124,7 @@ __device__ __forceinline__ QueueView queue_view(const Ctx& s, int slot, int cap)
   return q;
 }
 constexpr int QUEUE_BLOCKS = 4096;
+static inline int64_t red_groups_host(const Ctx& s) { return (s.N + WV_RED - 1) / WV_RED; }
 // sub-queue capacity of a launch of `items` threads in blocks of `block`
 static inline int queue_cap(int64_t items, int block) { return (int)(((items + block - 1) / block + 7) / 8) * block; }
 // MODE 0: op 17 (pressure geometry, gated), 1: op 23 (transport plane), 2: op 19 (export)
@@ -355,6 +356,74 @@ __device__ __forceinline__ float2 serial_sums(const float4* partial, int64_t gro
   return make_float2(acc, other);
 }
 
+// Up to eight in-order sums side by side, two per source array: chain 2 k
+// adds component ca[k] and chain 2 k + 1 component cb[k] (cb[k] < 0: none) of
+// src[k][0, groups) in index order (every src[k] readable; the caller ignores
+// the chains it does not need).  The
+// second wave stages the next chunk of every chain in LDS while lanes 0-7 of
+// the first wave add the current one, each lane its own chain in the same
+// instructions, in the loop of serial_sums.  Lane c of the first wave returns
+// chain c's sum.  Launched with WV_RED (two waves) threads.
+__device__ __forceinline__ float pick(float4 p, int c) { return c == 0 ? p.x : (c == 1 ? p.y : (c == 2 ? p.z : p.w)); }
+template <int NS>
+__device__ __forceinline__ float serial_sums_multi(const float4* const* src, const int* ca, const int* cb, int64_t groups) {
+  static_assert(WV_RED == 128 && NS <= 4, "one summing wave and one staging wave, at most 8 chains");
+  constexpr int CHUNK = 1024, NCH = 2 * NS, BATCH = 7, Q = CHUNK / 4;
+  __shared__ __align__(16) float stage[2][NCH][CHUNK];
+  const int t = threadIdx.x;
+  const int chunks = (int)((groups + CHUNK - 1) / CHUNK);
+  auto fill = [&](int c, int first, int stride) {
+    const int64_t base = (int64_t)c * CHUNK;
+    const int n = (int)(groups - base < CHUNK ? groups - base : CHUNK);
+    // every source's element loaded before any is staged: NS loads in flight per step
+#pragma unroll 2
+    for (int e = first; e < n; e += stride) {
+      float4 p[NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++) p[k] = src[k][base + e];
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        stage[c & 1][2 * k][e] = pick(p[k], ca[k]);
+        if (cb[k] >= 0) stage[c & 1][2 * k + 1][e] = pick(p[k], cb[k]);
+      }
+    }
+  };
+  float acc = 0;
+  if (chunks > 0) fill(0, t, WV_RED);
+  __syncthreads();
+  for (int c = 0; c < chunks; c++) {
+    if (t >= 64) {
+      if (c + 1 < chunks) fill(c + 1, t - 64, WV_RED - 64);
+    } else if (t < NCH) {
+      const float4* run = reinterpret_cast<const float4*>(stage[c & 1][t]);
+      const int64_t left = groups - (int64_t)c * CHUNK;
+      const int n = (int)(left < CHUNK ? left : CHUNK), n4 = n >> 2, nb = n4 / BATCH;
+      float4 a[BATCH], b[BATCH];
+#pragma unroll
+      for (int e = 0; e < BATCH; e++) a[e] = run[e];
+      for (int q = 0; q < nb; q += 2) {
+        const int jb = (q + 1) * BATCH < Q - BATCH ? (q + 1) * BATCH : Q - BATCH;
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) b[e] = run[jb + e];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) { acc += a[e].x; acc += a[e].y; acc += a[e].z; acc += a[e].w; }
+        if (q + 1 >= nb) break;
+        const int ja = (q + 2) * BATCH < Q - BATCH ? (q + 2) * BATCH : Q - BATCH;
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) a[e] = run[ja + e];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < BATCH; e++) { acc += b[e].x; acc += b[e].y; acc += b[e].z; acc += b[e].w; }
+      }
+      for (int j = nb * BATCH; j < n4; j++) { const float4 v = run[j]; acc += v.x; acc += v.y; acc += v.z; acc += v.w; }
+      for (int k = 4 * n4; k < n; k++) acc += stage[c & 1][t][k];
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 // ---------------------------------------------------- CFL / dt (20 + 21) -----
 __global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int final) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
@@ -448,11 +517,24 @@ __global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, int final) {
   if (threadIdx.x == 0) { s.c[39] += g.x; s.c[65] += g.y; s.c[66] += g.z; }
 }
 // ------------------------------------------------ VOF update (25 + 26) ------
+// Ctx::vpart: the VOF update's group partials of sweep k (k = c[21], at most
+// 3), then the sweep headers of the batch program
+__device__ __forceinline__ int64_t red_groups(const Ctx& s) { return (s.N + WV_RED - 1) / WV_RED; }
+__device__ __forceinline__ float4* vof_partials(const Ctx& s, int sweep) { return s.vpart + (int64_t)(sweep < 3 ? sweep : 3) * red_groups(s); }
+__device__ __forceinline__ float4* vof_header(const Ctx& s, int j) { return s.vpart + 4 * red_groups(s) + j; }
 // FINAL: the one-group finaliser, a kernel of its own so that the partial
-// pass does not carry the finaliser's LDS staging (occupancy)
+// pass does not carry the finaliser's LDS staging (occupancy).  Its argument
+// j < 0: the whole finaliser.  j = 0, 1, 2 (sweep j of the batch program):
+// the bounds and the failure test only, and header j = (ran, c[21], c[32]);
+// the two in-order ledger sums of the sweep (c[22], c[23] and the history's
+// z, w) are added by op 12's finaliser of the same batch (k_accept), in sweep
+// order.  Nothing between reads those, and they are added whether or not the
+// trial goes on to op 12's acceptance, as the per-sweep finaliser adds them.
 template <bool FINAL>
-__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int) {
-  if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
+__global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int j) {
+  const bool open = s.c[5] > .5f && s.c[7] < .5f;
+  if (FINAL && j >= 0 && threadIdx.x == 0) *vof_header(s, j) = make_float4(open ? 1 : 0, s.c[21], s.c[32], 0);
+  if (!open) return;
   if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(1e20f, -1e20f, 0, 0);
   int sweep = (int)s.c[21];
@@ -468,26 +550,34 @@ __global__ void __launch_bounds__(WV_RED) k_vof_update(Ctx s, int) {
     v = make_float4(f, f, change, fabsf(change)); if (isnan(f) || isinf(f)) { v.x = -1e30f; v.y = 1e30f; }
   }
   v = group_reduce<RED_BOUNDS>(v);
-  if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
+  if (threadIdx.x == 0) vof_partials(s, sweep)[blockIdx.x] = v;
   return;
   }
-  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  const int64_t G = red_groups(s);
+  const float4* part = vof_partials(s, (int)s.c[21]);
   // coupled_motion.glslinc:402-408: min/max in parallel, the two sums in index order
   float mn = 1e20f, mx = -1e20f;
 #pragma unroll 8
-  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, s.partial[k].x); mx = gmax(mx, s.partial[k].y); }
+  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { mn = gmin(mn, part[k].x); mx = gmax(mx, part[k].y); }
   float4 b = group_reduce<RED_BOUNDS>(make_float4(mn, mx, 0, 0));
-  const float2 zw = serial_sums<2, 3>(s.partial, G);
-  const float zs = zw.x, ws = zw.y;
+  float2 zw = make_float2(0, 0);
+  if (j < 0) zw = serial_sums<2, 3>(part, G);
   if (threadIdx.x == 0) {
     float* c = s.c; float cell = s.h[0] * s.h[1] * s.h[2];
     uint32_t step = (uint32_t)(int64_t)c[32] - 1u, sw = (uint32_t)c[21];
-    float4 hv = make_float4(b.x, b.y, zs, ws);
-    if (step < (uint32_t)s.hist_cap) s.history[(int64_t)step * 4 + sw] = hv;
-    c[22] += zs * cell; c[23] += ws * cell;
+    if (step < (uint32_t)s.hist_cap) {
+      float* hv = (float*)&s.history[(int64_t)step * 4 + sw];
+      hv[0] = b.x; hv[1] = b.y;
+      if (j < 0) { hv[2] = zw.x; hv[3] = zw.y; }
+    }
+    if (j < 0) { c[22] += zw.x * cell; c[23] += zw.y * cell; }
     if (b.x < -1e-6f || b.y > 1 + 1e-6f) { c[7] = 10 + c[21]; c[3]++; }
     c[21]++;
   }
+}
+void wv_launch_vof_deferred(const Ctx& s, int sweep, hipStream_t st) {
+  hipLaunchKernelGGL(k_vof_update<false>, dim3((unsigned)red_groups_host(s)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_vof_update<true>, dim3(1), dim3(WV_RED), 0, st, s, sweep);
 }
 // ---------------------------------------------- MacCormack (27 / 28) --------
 // coupled_motion.glslinc:168-188, 409-426 (transport-cache and plain forms)
@@ -972,9 +1062,10 @@ __global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, int final) {
 }
 // -------------------------------------- diagnostics + acceptance (12 + 13) --
 template <bool FINAL>
-__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int) {
-  if (!post_ok(s)) return;
+__global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int vof_ledgers) {
+  const bool ok = post_ok(s);
   if (!FINAL) {
+  if (!ok) return;
   int64_t i = GID(); float4 value = make_float4(0, 0, 0, 0); float dt = timestep(s);
   if (i < s.N) {
     int cb = candidate(s); I3 q = coord(i, dims(s)); float speed = 0;
@@ -993,14 +1084,45 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int) {
   if (threadIdx.x == 0) s.partial[blockIdx.x] = value;
   return;
   }
-  const int64_t G = (s.N + WV_RED - 1) / WV_RED;
+  const int64_t G = red_groups(s);
+  // vof_ledgers (the batch program): the VOF sweeps' in-order ledger sums
+  // left by their finalisers (k_vof_update), added first, in sweep order,
+  // whether or not the trial is accepted
+  float4 h[3];
+  bool pending = false;
+#pragma unroll
+  for (int j = 0; j < 3; j++) { h[j] = vof_ledgers ? *vof_header(s, j) : make_float4(0, 0, 0, 0); pending = pending || h[j].x > .5f; }
+  if (!ok && !pending) return;
   // pressure_pass.glslinc:78-103: maxima in parallel, the volume in index order
   float4 mx = make_float4(0, 0, 0, 0);
+  if (ok) {
 #pragma unroll 8
-  for (int64_t k = threadIdx.x; k < G; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
+    for (int64_t k = threadIdx.x; k < G; k += WV_RED) { float4 p = s.partial[k]; mx.x = gmax(mx.x, p.x); mx.y = gmax(mx.y, p.y); mx.z = gmax(mx.z, p.z); }
+  }
   mx = group_reduce<RED_DIAG>(mx);
-  const float vol = serial_sums<3, -1>(s.partial, G).x;
+  // chains 2 j, 2 j + 1: sweep j's z and w ledgers; chain 6: the volume
+  const float4* src[4];
+#pragma unroll
+  for (int j = 0; j < 3; j++) src[j] = vof_partials(s, h[j].x > .5f ? (int)h[j].y : 0);
+  src[3] = s.partial;
+  const int ca[4] = {2, 2, 2, 3}, cb[4] = {3, 3, 3, -1};
+  const float chain = serial_sums_multi<4>(src, ca, cb, G);
+  float sums[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) sums[k] = __shfl(chain, k, 64);
   if (threadIdx.x != 0) return;
+  {
+    float* c = s.c; const float cell = s.h[0] * s.h[1] * s.h[2];
+    for (int j = 0; j < 3; j++) {
+      if (h[j].x < .5f) continue;
+      const uint32_t step = (uint32_t)(int64_t)h[j].z - 1u, sw = (uint32_t)h[j].y;
+      if (step < (uint32_t)s.hist_cap) { float* hv = (float*)&s.history[(int64_t)step * 4 + sw]; hv[2] = sums[2 * j]; hv[3] = sums[2 * j + 1]; }
+      c[22] += sums[2 * j] * cell; c[23] += sums[2 * j + 1] * cell;
+      *vof_header(s, j) = make_float4(0, 0, 0, 0);
+    }
+  }
+  if (!ok) return;
+  const float vol = sums[6];
   float* c = s.c; float4 v = make_float4(mx.x, mx.y, mx.z, vol); float dt = timestep(s);
   c[14] = v.x; c[15] = v.y; c[16] = v.z; c[17] = v.w;
   if ((v.x >= 1e-4f || c[61] < .5f) && v.y < 1e20f && c[19] < .5f && c[58] < 4) { c[57] = 1; c[58]++; c[6] = 0; return; }
@@ -1014,6 +1136,10 @@ __global__ void __launch_bounds__(WV_RED) k_accept(Ctx s, int) {
   for (int field = 0; field < 6; field++) c[72 + field] += c[64 + field];
   c[90] = dt;
   c[0] = (float)candidate(s); c[1] += dt; c[2]++; c[4] = 0;
+}
+void wv_launch_accept(const Ctx& s, bool vof_ledgers, hipStream_t st) {
+  hipLaunchKernelGGL(k_accept<false>, dim3((unsigned)red_groups_host(s)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_accept<true>, dim3(1), dim3(WV_RED), 0, st, s, vof_ledgers ? 1 : 0);
 }
 // ------------------------------------------------ initial volume (41 + 42) --
 __global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, int final) {
@@ -1098,7 +1224,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 6: LRED(k_dot_alpha, 1); break;
   case 10: L256(k_direction, s.N); break;
   case 11: L256(k_project, s.NF); break;
-  case 12: LRED2(k_accept); break;
+  case 12: wv_launch_accept(s, false, st); break;
   case 14: L256(k_export_plain, s.N); break;
   case 16: L256(k_coefficients, s.N); break;
   case 17: launch_reconstruct<0>(s, st); break;
@@ -1112,7 +1238,10 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 22: L256(k_bank_copy, s.stride); break;
   case 23: launch_reconstruct<1>(s, st); break;
   case 24: L256(k_flux, s.NF); break;
-  case 25: LRED2(k_vof_update); break;
+  case 25:
+    hipLaunchKernelGGL(k_vof_update<false>, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+    hipLaunchKernelGGL(k_vof_update<true>, dim3(1), dim3(WV_RED), 0, st, s, -1);
+    break;
   case 27: case 28: {
     const bool corr = op == 28;
     if (!(s.flags & 4)) { if (corr) L256(k_maccormack<true>, s.NF); else L256(k_maccormack<false>, s.NF); break; }

@@ -286,6 +286,7 @@ extern "C" int wv_create(const wv_settings* settings, wv_solver** out) {
   ALLOC(c.dctl, 16);
   ALLOC(c.pacc, c.N);
   ALLOC(c.aux, 2 * c.N);
+  ALLOC(c.vpart, 4 * ((c.N + WV_RED - 1) / WV_RED) + 4);
   ALLOC(c.mq, c.NF + 8 * 256);   // the face count plus eight blocks (block-aggregated sub-queues)
   ALLOC(c.mq_count, 4 + 4 * 8);  // per queue: its length, then eight sub-queue counters
   ALLOC(c.reflect, 3 * (int64_t)(c.nx + 1) * (c.ny + 1));
@@ -380,7 +381,9 @@ static int record_batch(wv_solver* s) {
   auto d = [&](int op) { return dispatch(s, op); };
   d(43); d(1);
   d(20); d(21); d(45); d(46); d(22);
-  for (int sweep = 0; sweep < 3; sweep++) { d(23); d(24); d(55); d(56); d(25); d(26); }
+  // ops 25 + 26 of each sweep; the finalisers' in-order ledger sums are added
+  // by op 12's finaliser, all sweeps' and the volume's chains side by side
+  for (int sweep = 0; sweep < 3; sweep++) { d(23); d(24); d(55); d(56); wv_launch_vof_deferred(s->ctx, sweep, s->stream); }
   d(27); d(28);
   for (int op : {80, 81, 82, 83, 84, 87, 88}) d(op);
   d(17); d(18); d(16);
@@ -405,7 +408,7 @@ static int record_batch(wv_solver* s) {
   d(47);
   for (int op : {30, 31, 32, 33, 34, 47, 48}) d(op);
   d(35); d(36);
-  d(62); d(66); d(12); d(13);
+  d(62); d(66); wv_launch_accept(s->ctx, true, s->stream); d(13);
   wv_launch_step_record(s->ctx, s->stream);
   d(44); d(19);
   d(85); d(89);
