@@ -935,14 +935,23 @@ __global__ void __launch_bounds__(CHOL_THREADS) k_mg_cholesky(Ctx s, MGDev m, in
   // dense: the working matrix (lower triangle); lf: the factor; acc: rows of
   // the inverse still accumulating
   __shared__ float dense[64 * 64], lf[64 * 64], acc[64 * 64];
-  const int lid = threadIdx.x, N = m.lv[l].count; const signed char* slots = m.slots + 125 * l;
+  // the level's rows (k-major, width x N), active flags and delta slots staged
+  // first with independent loads, so the N^2 entries below read LDS only
+  __shared__ float rows[125 * 64];
+  __shared__ signed char sl125[128];
+  __shared__ unsigned char act[64];
+  const int lid = threadIdx.x, N = m.lv[l].count, W = m.lv[l].width;
+  if (l > 0) for (int k = lid; k < W * N; k += CHOL_THREADS) rows[k] = m.rows[m.lv[l].row_off + k];
+  for (int k = lid; k < 125; k += CHOL_THREADS) sl125[k] = m.slots[125 * l + k];
+  for (int k = lid; k < N; k += CHOL_THREADS) act[k] = mactive(m, l, k) ? 1 : 0;
+  __syncthreads();
   for (int k = lid; k < N * N; k += CHOL_THREADS) {
     int i = k / N, j = k % N; float v;
-    if (!mactive(m, l, i) || !mactive(m, l, j)) v = i == j ? 1 : 0;
+    if (!act[i] || !act[j]) v = i == j ? 1 : 0;
     else {
       int a = i < j ? i : j, b = i < j ? j : i; I3 d = mdelta(m, l, a, b);
-      int sl = in_box2(d) ? slots[slot125(d)] : -1;
-      v = sl < 0 ? 0 : m.rows[m.lv[l].row_off + (int64_t)sl * N + a];
+      int sl = in_box2(d) ? sl125[slot125(d)] : -1;
+      v = sl < 0 ? 0 : rows[sl * N + a];
     }
     dense[k] = v;
     acc[k] = i == j ? 1 : 0;

@@ -988,10 +988,18 @@ __global__ void __launch_bounds__(256) k_ext_hop(Ctx s, int rb) {
     const bool seam = axis == 2 ? (q.z == 0 || q.z == s.nz) : (axis == 0 && s.perx > .5f && (q.x == 0 || q.x == s.nx));
     if (source >= 0 && (int64_t)source == i && !seam) { s.motion[source_base(s, wb) + i] = source; return; }
     if (source >= 0) { I3 sq; int sa; unpack_face(s, (int64_t)source, sq, sa); F3 d = periodic_delta(s, q, sq); best = dot3(d, d); }
-    for (int ax = 0; ax < 3; ax++) for (int sign = -1; sign <= 1; sign += 2) {
-      I3 nb = q; addc(nb, ax, sign); nb = wrap_face(s, nb);
-      if (!face_in_bounds(s, nb, axis) || wall_face(s, nb, axis)) continue;
-      float cs = s.motion[source_base(s, rb) + face_local(s, nb, axis)]; if (cs < 0) continue;
+    // the six neighbours' sources loaded together first; the wall test (two
+    // bed evaluations) only for a neighbour that has a source, in the same
+    // (axis, sign) order: the same candidates as testing the wall first
+    I3 nbs[6]; float css[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      I3 nb = q; addc(nb, k >> 1, (k & 1) ? 1 : -1); nb = wrap_face(s, nb); nbs[k] = nb;
+      css[k] = face_in_bounds(s, nb, axis) ? s.motion[source_base(s, rb) + face_local(s, nb, axis)] : -1.0f;
+    }
+    for (int k = 0; k < 6; k++) {
+      const float cs = css[k];
+      if (cs < 0 || wall_face(s, nbs[k], axis)) continue;
       I3 sq; int sa; unpack_face(s, (int64_t)cs, sq, sa); F3 d = periodic_delta(s, q, sq); float distance = dot3(d, d);
       if (distance < best || (distance == best && cs < source)) { best = distance; source = cs; }
     }
