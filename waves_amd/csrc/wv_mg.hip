@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict_coop(Ctx s, MGDev m, int l,
       if (!P.aggregate) {
         I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z};
         for (int a = 0; a < 3; a++) {
-          float pos = ((float)qa[a] + .5f) / (float)P.f[a] - .5f; int base = (int)floorf(pos); float tt = fract(pos);
+          float pos = lattice_pos(qa[a], P.f[a]); int base = (int)floorf(pos); float tt = fract(pos);
           float w = 0;
           for (int bit = 0; bit < P.f[a]; bit++) {
             int nb = base + bit;
@@ -338,7 +338,7 @@ __device__ __forceinline__ void prolong_row(const MGDev& m, int l, int ff, int c
     const bool on = m.vec[at(m, l, 3, i)] > .5f;  // mactive(m, l, i)
     const I3 n = mdims(m, l + 1), q = coord(i, mdims(m, l));
     int base[3]; float t[3]; const int qq[3] = {q.x, q.y, q.z};
-    for (int k = 0; k < 3; k++) { float pos = ((float)qq[k] + .5f) / (float)L.f[k] - .5f; base[k] = (int)floorf(pos); t[k] = fract(pos); }
+    for (int k = 0; k < 3; k++) { float pos = lattice_pos(qq[k], L.f[k]); base[k] = (int)floorf(pos); t[k] = fract(pos); }
     float wc[8], fc[8], vc[8]; bool use[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -638,12 +638,16 @@ __global__ void __launch_bounds__(256) k_mg_l1_coarse(Ctx s, MGDev m, int l) {
 //   C[I, J] = sum_i P[i, I] T_i[J]
 // over the fine nodes i whose transfers reach I.  Both stages run their sums in
 // a fixed order: the coarse operator is deterministic.
-__device__ __forceinline__ int floordiv(int a, int b) { int q = a / b; return (a % b != 0 && (a < 0) != (b < 0)) ? q - 1 : q; }
+__device__ __forceinline__ int floordiv(int a, int b) {
+  if (b == 1) return a;
+  if (b == 2) return a >> 1;  // arithmetic shift: the floor of a / 2
+  int q = a / b; return (a % b != 0 && (a < 0) != (b < 0)) ? q - 1 : q;
+}
 // first (unclamped) parent coordinate of fine coordinate j along axis a
 __device__ __forceinline__ int parent_base(const MGLevel& F, int a, int j) {
   if (F.aggregate) return floordiv(j, F.f[a]);
   if (F.f[a] == 1) return j;
-  return (int)floorf(((float)j + .5f) / (float)F.f[a] - .5f);
+  return (int)floorf(lattice_pos(j, F.f[a]));
 }
 // coarse node of an unclamped parent coordinate (pressure_multigrid.glslinc:115-117),
 // -1 if dropped (beyond the top)
@@ -691,7 +695,7 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l, i
     int base[3]; float t[3];
     for (int a = 0; a < 3; a++) {
       if (F.aggregate || F.f[a] == 1) { base[a] = parent_base(F, a, ju[a]); t[a] = 0; }
-      else { float pos = ((float)ju[a] + .5f) / (float)F.f[a] - .5f; base[a] = (int)floorf(pos); t[a] = fract(pos); }
+      else { float pos = lattice_pos(ju[a], F.f[a]); base[a] = (int)floorf(pos); t[a] = fract(pos); }
     }
     // at most 2 x 2 x 2 parents (coarsening factors are 1 or 2): their
     // indices and weights first, then all their flags loaded together, then
@@ -1162,7 +1166,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
           if (!P.aggregate) {
             I3 q = coord(j, nf); int qa[3] = {q.x, q.y, q.z};
             for (int a = 0; a < 3; a++) {
-              float pos = ((float)qa[a] + .5f) / (float)P.f[a] - .5f; int base = (int)floorf(pos); float tt = fract(pos);
+              float pos = lattice_pos(qa[a], P.f[a]); int base = (int)floorf(pos); float tt = fract(pos);
               float w = 0;
               for (int bit = 0; bit < P.f[a]; bit++) {
                 int nb = base + bit;
@@ -1190,11 +1194,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_mg_tail(Ctx s, MGDev m, TailLa
       if (active(l, i)) {
         I3 q = coord(i, nlv);
         if (L.aggregate) {
-          int j = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), n1);
+          int j = (int)index3(mk(fdiv(q.x, L.f[0]), fdiv(q.y, L.f[1]), fdiv(q.z, L.f[2])), n1);
           if (active(l + 1, j)) add += 1 * vec(l + 1, cf)[j];
         } else {
           int base[3]; float t[3]; int qq[3] = {q.x, q.y, q.z};
-          for (int a = 0; a < 3; a++) { float pos = ((float)qq[a] + .5f) / (float)L.f[a] - .5f; base[a] = (int)floorf(pos); t[a] = fract(pos); }
+          for (int a = 0; a < 3; a++) { float pos = lattice_pos(qq[a], L.f[a]); base[a] = (int)floorf(pos); t[a] = fract(pos); }
           for (int z = 0; z < L.f[2]; z++) for (int y = 0; y < L.f[1]; y++) for (int x = 0; x < L.f[0]; x++) {
             int bit[3] = {x, y, z}; I3 nb = mk(base[0] + x, base[1] + y, base[2] + z); float ww = 1;
             for (int a = 0; a < 3; a++) if (L.f[a] > 1) ww *= bit[a] == 1 ? t[a] : 1 - t[a];
@@ -1326,7 +1330,7 @@ __global__ void __launch_bounds__(256) k_mg_tail_tables(MGDev m, TailFast t, int
       const MGLevel& L = m.lv[l];
       if (e < t.par[l] || e >= t.par[l] + L.count) continue;
       const I3 q = coord(e - t.par[l], mdims(m, l));
-      v = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), mdims(m, l + 1));
+      v = (int)index3(mk(fdiv(q.x, L.f[0]), fdiv(q.y, L.f[1]), fdiv(q.z, L.f[2])), mdims(m, l + 1));
     }
     tab[e] = (int16_t)v;
   }

@@ -55,15 +55,25 @@ __device__ __forceinline__ float mapply(const MGDev& m, int l, int field, int i)
   }
   return sum;
 }
+// The transfers' lattice position (float(j) + .5) / float(f) - .5 of fine
+// coordinate j (pressure_multigrid.glslinc:108-120) for a coarsening factor f
+// of 1 or 2: a division by a power of two is exact, so it is the product with
+// 1 / f (the same value, without the division's instruction sequence)
+__device__ __forceinline__ float lattice_pos(int j, int f) {
+  const float x = (float)j + .5f;
+  return (f == 2 ? x * .5f : (f == 1 ? x : x / (float)f)) - .5f;
+}
+// q / f for a coordinate q >= 0 and a factor f of 1 or 2 (a shift), else the division
+__device__ __forceinline__ int fdiv(int q, int f) { return f == 2 ? q >> 1 : (f == 1 ? q : q / f); }
 // pressure_multigrid.glslinc:108-120: transfers of fine node i (level l) to level l+1
 __device__ __forceinline__ int mtransfers(const MGDev& m, int l, int i, int* ids, float* w) {
   const MGLevel& L = m.lv[l]; I3 n = mdims(m, l + 1), q = coord(i, mdims(m, l));
   if (L.aggregate) {
-    int j = (int)index3(mk(q.x / L.f[0], q.y / L.f[1], q.z / L.f[2]), n);
+    int j = (int)index3(mk(fdiv(q.x, L.f[0]), fdiv(q.y, L.f[1]), fdiv(q.z, L.f[2])), n);
     if (!mactive(m, l + 1, j)) return 0; ids[0] = j; w[0] = 1; return 1;
   }
   int base[3]; float t[3]; int qq[3] = {q.x, q.y, q.z};
-  for (int k = 0; k < 3; k++) { float pos = ((float)qq[k] + .5f) / (float)L.f[k] - .5f; base[k] = (int)floorf(pos); t[k] = fract(pos); }
+  for (int k = 0; k < 3; k++) { float pos = lattice_pos(qq[k], L.f[k]); base[k] = (int)floorf(pos); t[k] = fract(pos); }
   int size = 0;
   for (int z = 0; z < L.f[2]; z++) for (int y = 0; y < L.f[1]; y++) for (int x = 0; x < L.f[0]; x++) {
     int bit[3] = {x, y, z}; I3 nb = mk(base[0] + x, base[1] + y, base[2] + z); float ww = 1;
