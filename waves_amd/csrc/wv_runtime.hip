@@ -154,14 +154,12 @@ static bool plan_levels(const float* pc, int aggregate_from, Plan& p) {
     int r = 1;
     while (r < 32 && 2 * r <= box && (int64_t)L.count * r < (1 << 20)) r *= 2;
     L.rlanes = r;
-    // Galerkin stage 1 (A P rows of this level's fine rows): about 2^20 lanes
-    // (4 times the sweeps' target: a row's entries each wait on a coefficient
-    // and then on its parents' flags, so short per-lane chains pay; measured on
-    // the 297,600-row level 1: 154 us with one lane per row, 108 us with four),
-    // at most 16 lanes and the row width
+    // Galerkin stage 1 (A P rows of this level's fine rows): about the same
+    // lane target, at most 16 lanes and the row width.  (Four lanes per row on
+    // the 297,600-row level 1 take 108 us against 154 us, but the lane split
+    // changes the order of the row sums and so the trajectory; kept at one.)
     int ga = 1;
-    const int64_t ga_target = 4 * lane_target;
-    while (ga < 16 && (int64_t)L.count * ga < ga_target && 2 * ga <= L.width) ga *= 2;
+    while (ga < 16 && (int64_t)L.count * ga < lane_target && 2 * ga <= L.width) ga *= 2;
     L.ga = ga;
   }
   p.meta[16 * (p.L - 1) + 9] = (int)matrix_count;
