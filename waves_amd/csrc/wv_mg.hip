@@ -909,15 +909,24 @@ __global__ void __launch_bounds__(256) k_mg_symmetrize(Ctx s, MGDev m, int l) {
   const MGLevel& C = m.lv[l];
   const uint32_t gid = (uint32_t)GID(), n = (uint32_t)C.count;
   const int lane = (int)(gid / n); const int64_t I = gid - (uint32_t)lane * n; if (lane >= SYM_LANES) return;
-  I3 q = coord(I, mdims(m, l));
+  const I3 nd = mdims(m, l), q = coord(I, nd);
   float* base = m.rows + C.row_off;
   for (int k = lane; k < C.width; k += SYM_LANES) {
     I3 off = moffset(m, l, k); int J = mwrap(m, l, mk(q.x + off.x, q.y + off.y, q.z + off.z));
     if (J < 0) { base[(int64_t)k * C.count + I] = 0; continue; }
-    I3 d = mdelta(m, l, (int)I, J);
+    // a neighbour reached without a wrap lies at coordinate q + off, so
+    // mdelta(I, J) is the minimal image of off and mdelta(J, I) that of -off
+    const bool direct = (unsigned)(q.x + off.x) < (unsigned)nd.x && (unsigned)(q.z + off.z) < (unsigned)nd.z;
+    I3 d, e;
+    if (direct) {
+      d = off; e = mk(-off.x, -off.y, -off.z);
+      if (m.perx) { d.x = pmod(d.x + nd.x / 2, nd.x) - nd.x / 2; e.x = pmod(e.x + nd.x / 2, nd.x) - nd.x / 2; }
+      d.z = pmod(d.z + nd.z / 2, nd.z) - nd.z / 2; e.z = pmod(e.z + nd.z / 2, nd.z) - nd.z / 2;
+    } else d = mdelta(m, l, (int)I, J);
     if (d.x != off.x || d.y != off.y || d.z != off.z) { base[(int64_t)k * C.count + I] = 0; continue; }
     if (J < I) {
-      I3 e = mdelta(m, l, J, (int)I); int sl = slots[slot125(e)];
+      if (!direct) e = mdelta(m, l, J, (int)I);
+      const int sl = slots[slot125(e)];
       base[(int64_t)k * C.count + I] = base[(int64_t)sl * C.count + J];
     }
   }
