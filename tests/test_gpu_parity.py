@@ -339,3 +339,56 @@ def test_mg_invalid_coefficients(gpu):
     P.check_group(("build",), rep)
     assert pair.o.control[7] == 34
     pair.close()
+
+
+@pytest.mark.parametrize("fail_sweep", [False, True])
+def test_batch_program_matches_ops(gpu, tmp_path, fail_sweep):
+    """The batch program (one hipGraph replay: fused finalisers, the VOF
+    sweeps' in-order ledger sums added by op 12's finaliser) against the same
+    batches launched op by op through wv_op, on one device and identical
+    inputs: every field bit-identical after every batch, up to the accepted
+    step.  F = 1 + 5e-7 in one wet cell gives the sweeps nonzero clamp
+    ledgers; fail_sweep: F = 1.25 in another makes sweep 0 fail (code 10), so
+    the ledger sums of a failed trial are added too
+    (coupled_motion.glslinc:402-408)."""
+    from waves_amd import solver as S
+    s = narrow_flume(**SMALL)
+    a = S.Solver(s)
+    a.start()
+    for _ in range(2):
+        a.step()
+    path = str(tmp_path / "state.wvs")
+    a.save_state(path)
+    b = S.Solver(s)
+    b.load_state(path)
+    c = a.control
+    acc = int(c[0])
+    F = a.read(acc)
+    wet = np.nonzero(a.read(S.FIELD_GEO)[1::2] > 0.5)[0]
+    F[int(wet[len(wet) // 3])] = np.float32(1 + 5e-7)
+    if fail_sweep:
+        F[int(wet[2 * len(wet) // 3])] = 1.25
+    a.write(acc, F)
+    b.write(acc, F)
+    target = int(c[2]) + 1
+    a.request(target)
+    b.request(target)
+    for batch in range(4):
+        a.frame(1)
+        ca = a.poll(wait=True)
+        for grp in P.batch_groups(s.iterations_per_batch):
+            P.run_gpu(b, grp)
+        cb = b.control
+        say("fail_sweep", fail_sweep, "batch", batch, "c[7]", ca[7], cb[7], "c[22..25]", ca[22:26], cb[22:26],
+            "accepted", ca[2], cb[2])
+        for f in P.STATE_FIELDS:
+            x, y = a.read(f), b.read(f)
+            nd = int((x.view(np.uint32) != y.view(np.uint32)).sum())
+            assert nd == 0, (batch, P.NAMES[f], nd)
+        if ca[2] >= target or ca[7] > 0:
+            break
+    assert ca[7] == (10 if fail_sweep else 0)
+    assert fail_sweep or ca[2] == target
+    assert ca[22] != 0 or ca[24] != 0   # the clamp ledgers were formed (c[22] moves into c[24] on acceptance)
+    a.close()
+    b.close()
