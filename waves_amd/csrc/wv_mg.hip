@@ -374,9 +374,10 @@ __global__ void __launch_bounds__(256) k_mg_coarse_solve(Ctx s, MGDev m, int l, 
 }
 // pressure_multigrid.glslinc:230-234 (op 10) fused with the PCG finaliser:
 // phase 0 -> pressure_iterations.glslinc:14-19 (op 4), phase 1 -> :31 (op 9)
-__global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, int phase, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, int phase, int) {
   if (!mg_enabled(s, phase)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); int64_t N = s.N; float4 value = make_float4(0, 0, 0, 0);
   if (i < N) {
     float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i];
@@ -406,10 +407,11 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out(Ctx s, MGDev m, int field, in
 }
 // op 3 when the V-cycle is predicated off (a failure code is set): the
 // reference's op 4 then reduces op 3's own partials.
-__global__ void __launch_bounds__(WV_RED) k_pcg_init_partials(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_pcg_init_partials(Ctx s, int) {
   if (!pressure_setup(s)) return;
   if (mg_enabled(s, 0)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); int64_t N = s.N; float4 v = make_float4(0, 0, 0, 0);
   if (i < N) { float r = s.work[i], z = s.work[N + i], bv = s.work[5 * N + i]; v = make_float4(r * z, r * r, bv * bv, fabsf(r)); }
   v = group_reduce<RED_PRESSURE>(v);
@@ -1613,14 +1615,14 @@ void wv_mg_cycle(const Ctx& s, const MGDev& m, int phase, int smoothing, bool lo
   if (load) hipLaunchKernelGGL(k_mg_load, dim3(nblocks(m.lv[0].count, 256)), dim3(256), 0, st, s, m, phase, (int)first_sweep_fused(m, 0));
   CycleBuilder<false> b{s, m, phase, smoothing, st};
   int final_field = b.cycle(0);
-  hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
-  hipLaunchKernelGGL(k_mg_out, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
+  hipLaunchKernelGGL(k_mg_out<false>, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
+  hipLaunchKernelGGL(k_mg_out<true>, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
 }
 
 // op 3's partials + op 4 when the V-cycle is predicated off
 void wv_launch_pcg_init_partials(const Ctx& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_pcg_init_partials, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
-  hipLaunchKernelGGL(k_pcg_init_partials, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_pcg_init_partials<false>, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_pcg_init_partials<true>, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
 
 // Hot kernels launched on their own for live timing (the kernels' outputs are
@@ -1690,7 +1692,7 @@ void wv_mg_cycle_bf16(const Ctx& s, const MGDev& m, int phase, int smoothing, bo
   int final_field = b.cycle(0);
   if (fp64_out) wv_launch_mg_out64(s, m, final_field, phase, st);
   else {
-    hipLaunchKernelGGL(k_mg_out, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
-    hipLaunchKernelGGL(k_mg_out, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
+    hipLaunchKernelGGL(k_mg_out<false>, dim3(nblocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, final_field, phase, 0);
+    hipLaunchKernelGGL(k_mg_out<true>, dim3(1), dim3(WV_RED), 0, st, s, m, final_field, phase, 1);
   }
 }

@@ -425,9 +425,10 @@ __device__ __forceinline__ float serial_sums_multi(const float4* const* src, con
 }
 
 // ---------------------------------------------------- CFL / dt (20 + 21) -----
-__global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID();
   float rate = i < s.N ? courant_at(s, coord(i, dims(s)), accepted(s), true) : 0;
   float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
@@ -448,9 +449,10 @@ __global__ void __launch_bounds__(WV_RED) k_cfl(Ctx s, int final) {
   }
 }
 // ---------------------------------------------- compression audit (45 + 46) --
-__global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_compression(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     int acc = accepted(s); float dt = timestep(s); float cell = s.h[0] * s.h[1] * s.h[2];
@@ -498,9 +500,10 @@ __global__ void __launch_bounds__(256) k_flux(Ctx s) {
   s.motion[flux_base(s) + i] = amount;
 }
 // ---------------------------------------------- boundary ledgers (55 + 56) --
-__global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_ledgers(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     int axis = ((int)s.c[2] + (int)s.c[21]) % 3;
@@ -683,9 +686,10 @@ __device__ __forceinline__ F3 cell_velocity(const Ctx& s, int bank, I3 q) {
 }
 __device__ __forceinline__ bool offshore_zone(const Ctx& s, float x) { return x < s.forc[0] + s.forc[1]; }
 // coastal_pass.glslinc:46-58 (ops 80 + 81)
-__global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_relax_f(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     I3 q = coord(i, dims(s));
@@ -723,9 +727,10 @@ __global__ void __launch_bounds__(256) k_relax_faces(Ctx s) {
   putv(s, cb, U_(s) + i, value); s.motion[forward_base(s) + i] = value - old;
 }
 // coastal_pass.glslinc:69-77 (ops 83 + 84)
-__global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float4 v = make_float4(0, 0, 0, 0);
   if (i < s.N) {
     I3 q = coord(i, dims(s)); F3 ch; float cell = s.h[0] * s.h[1] * s.h[2];
@@ -742,9 +747,10 @@ __global__ void __launch_bounds__(WV_RED) k_momentum(Ctx s, int final) {
   if (threadIdx.x == 0) { s.c[67] += g.x; s.c[68] += g.y; s.c[69] += g.z; }
 }
 // coastal_pass.glslinc:84-92 (ops 87 + 88)
-__global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_forcing_bounds(Ctx s, int) {
   if (!(s.c[5] > .5f && s.c[7] < .5f)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID();
   float f = i < s.N ? getv(s, candidate(s), i) : 0;
   float4 v = (isnan(f) || isinf(f)) ? make_float4(-1e30f, 1e30f, 0, 0) : make_float4(f, f, 0, 0);
@@ -903,9 +909,10 @@ __global__ void __launch_bounds__(WV_RED) k_spmv_dot(Ctx s) {
   if (threadIdx.x == 0) s.partial[blockIdx.x] = v;
 }
 // pressure_iterations.glslinc:21-22 (ops 6 + 7)
-__global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_dot_alpha(Ctx s, int) {
   if (!solving(s)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); int64_t N = s.N;
   float4 v = i < N ? make_float4(s.work[2 * N + i] * s.work[3 * N + i], 0, 0, 0) : make_float4(0, 0, 0, 0);
   v = group_reduce<RED_SUM>(v);
@@ -1034,9 +1041,10 @@ __global__ void __launch_bounds__(256) k_masks(Ctx s) {
   }
 }
 // --------------------------------------------- post-projection CFL (35+36) --
-__global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, int) {
   if (!post_ok(s)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID();
   float rate = i < s.N ? courant_at(s, coord(i, dims(s)), candidate(s), false) * timestep(s) : 0;
   float4 v = group_reduce<RED_BOUNDS>(make_float4(rate, rate, 0, 0));
@@ -1051,9 +1059,10 @@ __global__ void __launch_bounds__(WV_RED) k_post_courant(Ctx s, int final) {
   if (threadIdx.x == 0) s.c[27] = r.x;
 }
 // ------------------------------------------------ true residual (62 + 66) --
-__global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_true_residual(Ctx s, int) {
   if (!post_ok(s)) return;
-  if (!final) {
+  if (!FINAL) {
   int64_t i = GID(); float r = 0;
   if (i < s.N) { I3 q = coord(i, dims(s)); if (wet(s, q)) r = 1000 / timestep(s) * divergence(s, q); }
   float4 v = group_reduce<RED_SUM>(make_float4(r * r, 0, 0, 0));
@@ -1150,8 +1159,9 @@ void wv_launch_accept(const Ctx& s, bool vof_ledgers, hipStream_t st) {
   hipLaunchKernelGGL(k_accept<true>, dim3(1), dim3(WV_RED), 0, st, s, vof_ledgers ? 1 : 0);
 }
 // ------------------------------------------------ initial volume (41 + 42) --
-__global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, int final) {
-  if (!final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_volume(Ctx s, int) {
+  if (!FINAL) {
   int64_t i = GID();
   float4 v = i < s.N ? make_float4(getv(s, accepted(s), i) * s.h[0] * s.h[1] * s.h[2], 0, 0, 0) : make_float4(0, 0, 0, 0);
   v = group_reduce<RED_SUM>(v);
@@ -1229,7 +1239,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 2: L256(k_gravity_prep, s.stride); break;
   case 3: L256(k_pcg_init, s.N); break;
   case 5: L256(k_spmv, s.N); break;
-  case 6: LRED(k_dot_alpha, 1); break;
+  case 6: LRED2(k_dot_alpha); break;
   case 10: L256(k_direction, s.N); break;
   case 11: L256(k_project, s.NF); break;
   case 12: wv_launch_accept(s, false, st); break;
@@ -1242,7 +1252,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
     hipLaunchKernelGGL(k_distance_deferred, dim3(QUEUE_BLOCKS * 256 / DIST_THREADS), dim3(DIST_THREADS), 0, st, s, queue_cap(s.N, 64));
     break;
   case 19: launch_reconstruct<2>(s, st); break;
-  case 20: LRED(k_cfl, 3); break;
+  case 20: LRED2(k_cfl); break;
   case 22: L256(k_bank_copy, s.stride); break;
   case 23: launch_reconstruct<1>(s, st); break;
   case 24: L256(k_flux, s.NF); break;
@@ -1265,24 +1275,24 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
   case 30: L256(k_ext_seed, s.NF); break;
   case 31: case 32: case 33: hipLaunchKernelGGL(k_ext_hop, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s, (op - 31) % 2); break;  // one wave per block
   case 34: L256(k_ext_fill, s.NF); break;
-  case 35: LRED(k_post_courant, 5); break;
-  case 41: LRED(k_volume, 6); break;
+  case 35: LRED2(k_post_courant); break;
+  case 41: LRED2(k_volume); break;
   case 44: hipLaunchKernelGGL(k_history, dim3(1), dim3(1), 0, st, s); break;
-  case 45: LRED(k_compression, 7); break;
+  case 45: LRED2(k_compression); break;
   case 47: L256(k_periodic_sync, (int64_t)s.nx * s.ny + (int64_t)s.ny * s.nz); break;  // the seams' duplicate faces
   case 48: L256(k_masks<false>, s.NF > s.N ? s.NF : s.N); break;
   case 49: L256(k_masks<true>, s.NF > s.N ? s.NF : s.N); break;
-  case 55: LRED(k_ledgers, 8); break;
-  case 62: LRED(k_true_residual, 9); break;
+  case 55: LRED2(k_ledgers); break;
+  case 62: LRED2(k_true_residual); break;
   case 63: hipLaunchKernelGGL(k_defect_state, dim3(1), dim3(1), 0, st, s); break;
   case 64: L256(k_defect_save, s.N); break;
   case 65: L256(k_defect_restore, s.N); break;
-  case 80: LRED(k_relax_f, 10); break;
+  case 80: LRED2(k_relax_f); break;
   case 82: hipLaunchKernelGGL(k_relax_faces, dim3(blocks(s.NF, 64)), dim3(64), 0, st, s); break;  // one wave per block
-  case 83: LRED(k_momentum, 11); break;
+  case 83: LRED2(k_momentum); break;
   case 85: hipLaunchKernelGGL(k_gauges, dim3(8), dim3(256), 0, st, s); break;
   case 86: hipLaunchKernelGGL(k_probes, dim3(1), dim3(64), 0, st, s); break;
-  case 87: LRED(k_forcing_bounds, 12); break;
+  case 87: LRED2(k_forcing_bounds); break;
   case 89: hipLaunchKernelGGL(k_coastal_history, dim3(1), dim3(1), 0, st, s); break;
   // finalisers fused into the kernel of the preceding op, and PCG reductions
   // fused into the V-cycle output kernel (op 4 and op 9)
@@ -1296,7 +1306,7 @@ int wv_launch_op(const Ctx& s, int op, hipStream_t st) {
 // ops 5, 6 and 7 of one PCG iteration: A p with the p.Ap partials, then alpha
 void wv_launch_spmv_dot(const Ctx& s, hipStream_t st) {
   hipLaunchKernelGGL(k_spmv_dot, dim3(blocks(s.N, WV_RED)), dim3(WV_RED), 0, st, s);
-  hipLaunchKernelGGL(k_dot_alpha, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_dot_alpha<true>, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
 void wv_launch_pcg_update(const Ctx& s, const MGDev& m, hipStream_t st) {
   hipLaunchKernelGGL(k_pcg_update, dim3(blocks(s.N, 256)), dim3(256), 0, st, s, m, (int)wv_mg_first_sweep_fused(m));
