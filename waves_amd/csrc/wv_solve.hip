@@ -63,8 +63,9 @@ void wv_solve_begin(const Ctx& s, bool zero_pressure, hipStream_t st) {
 }
 
 // zero-mean right-hand side over the wet cells (p = 0, so r = b, z = r / diag)
-__global__ void __launch_bounds__(WV_RED) k_rhs_mean(Ctx s, int final) {
-  if (!final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_rhs_mean(Ctx s, int) {
+  if (!FINAL) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
     if (i < s.N && wet(s, coord(i, dims(s)))) v = mkd(s.work[5 * s.N + i], 1, 0, 0);
     v = group_reduce_d(v); put_partial(s, v); return;
@@ -80,16 +81,17 @@ __global__ void __launch_bounds__(256) k_rhs_shift(Ctx s) {
   s.work[5 * N + i] = b; s.work[i] = b; s.work[N + i] = z; s.work[2 * N + i] = z;
 }
 void wv_solve_zero_mean(const Ctx& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_rhs_mean, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
-  hipLaunchKernelGGL(k_rhs_mean, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_rhs_mean<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_rhs_mean<true>, dim3(1), dim3(WV_RED), 0, st, s, 1);
   hipLaunchKernelGGL(k_rhs_shift, dim3(nb(s.N, 256)), dim3(256), 0, st, s);
 }
 
 // ops 6 + 7 with an FP64 dot product: alpha = rho / (p . A p)
-__global__ void __launch_bounds__(WV_RED) k_dot_alpha64(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_dot_alpha64(Ctx s, int) {
   if (!solving(s)) return;
   int64_t N = s.N;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID();
     d4 v = i < N ? mkd((double)s.work[2 * N + i] * (double)s.work[3 * N + i], 0, 0, 0) : mkd(0, 0, 0, 0);
     v = group_reduce_d(v); put_partial(s, v); return;
@@ -102,15 +104,16 @@ __global__ void __launch_bounds__(WV_RED) k_dot_alpha64(Ctx s, int final) {
   }
 }
 void wv_solve_dot_alpha64(const Ctx& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot_alpha64, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
-  hipLaunchKernelGGL(k_dot_alpha64, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_dot_alpha64<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_dot_alpha64<true>, dim3(1), dim3(WV_RED), 0, st, s, 1);
 }
 
 // MG op 10 + op 4 / op 9 with FP64 sums (pressure_iterations.glslinc:14-19, 31)
-__global__ void __launch_bounds__(WV_RED) k_mg_out64(Ctx s, MGDev m, int field, int phase, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_mg_out64(Ctx s, MGDev m, int field, int phase, int) {
   if (!mg_enabled(s, phase)) return;
   int64_t N = s.N;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
     if (i < N) {
       float r = s.work[i], z = m.vec[at(m, 0, field, i)], b = s.work[5 * N + i], zold = s.work[N + i];
@@ -143,8 +146,8 @@ __global__ void __launch_bounds__(WV_RED) k_mg_out64(Ctx s, MGDev m, int field, 
   }
 }
 void wv_launch_mg_out64(const Ctx& s, const MGDev& m, int field, int phase, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_out64, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, field, phase, 0);
-  hipLaunchKernelGGL(k_mg_out64, dim3(1), dim3(WV_RED), 0, st, s, m, field, phase, 1);
+  hipLaunchKernelGGL(k_mg_out64<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, m, field, phase, 0);
+  hipLaunchKernelGGL(k_mg_out64<true>, dim3(1), dim3(WV_RED), 0, st, s, m, field, phase, 1);
 }
 
 // True residual r = b - A p in FP64 from the FP32 pressure and coefficients
@@ -169,9 +172,10 @@ __device__ __forceinline__ double true_residual_at(const Ctx& s, int64_t i, int 
   }
   return (double)s.work[5 * N + i] - ap;
 }
-__global__ void __launch_bounds__(WV_RED) k_true_residual64(Ctx s, int final, int replace, int parts) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_true_residual64(Ctx s, int, int replace, int parts) {
   int64_t N = s.N;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
     if (i < N) { double r = true_residual_at(s, i, parts), b = s.work[5 * N + i]; v = mkd(r * r, b * b, 0, fabs(r)); }
     v = group_reduce_d(v); put_partial(s, v); return;
@@ -197,8 +201,8 @@ __global__ void __launch_bounds__(256) k_replace_residual(Ctx s) {
   s.work[i] = (float)true_residual_at(s, i, 3);
 }
 void wv_solve_true_residual64(const Ctx& s, const MGDev& m, bool replace, int parts, hipStream_t st) {
-  hipLaunchKernelGGL(k_true_residual64, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0, replace ? 1 : 0, parts);
-  hipLaunchKernelGGL(k_true_residual64, dim3(1), dim3(WV_RED), 0, st, s, 1, replace ? 1 : 0, parts);
+  hipLaunchKernelGGL(k_true_residual64<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0, replace ? 1 : 0, parts);
+  hipLaunchKernelGGL(k_true_residual64<true>, dim3(1), dim3(WV_RED), 0, st, s, 1, replace ? 1 : 0, parts);
   if (replace) {
     hipLaunchKernelGGL(k_fold_pressure, dim3(nb(s.N, 256)), dim3(256), 0, st, s);
     hipLaunchKernelGGL(k_replace_residual, dim3(nb(s.N, 256)), dim3(256), 0, st, s);
@@ -234,9 +238,10 @@ void wv_solve_init_given(const Ctx& s, hipStream_t st) {
 
 // ------------------------------------------------- learned warm start ----
 // Network input per cell: F, r / rms_wet(r), p / rms_wet(p), wet, 0 x 4 (bf16).
-__global__ void __launch_bounds__(WV_RED) k_net_scale(Ctx s, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_net_scale(Ctx s, int) {
   int64_t N = s.N;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0);
     if (i < N && wet(s, coord(i, dims(s)))) {
       double r = s.work[i], p = getv(s, candidate(s), P_(s) + i);
@@ -282,9 +287,10 @@ __device__ __forceinline__ float ws_apply(const Ctx& s, const float* qf, const f
 }
 // Galerkin correction on span{v1, v2}: minimise the A-norm error of p + a v1 + c v2,
 // [v1 A v1, v1 A v2; v2 A v1, v2 A v2] [a; c] = [v1 . r; v2 . r], in FP64
-__global__ void __launch_bounds__(WV_RED) k_ws_dots(Ctx s, const float* qf, const float* qc, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_ws_dots(Ctx s, const float* qf, const float* qc, int) {
   int64_t N = s.N, groups = (N + WV_RED - 1) / WV_RED;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID(); d4 v = mkd(0, 0, 0, 0), u = mkd(0, 0, 0, 0);
     if (i < N) {
       I3 q = coord(i, dims(s));
@@ -317,13 +323,13 @@ __global__ void __launch_bounds__(256) k_ws_update(Ctx s, const float* qf, const
   putv(s, cb, k, (float)((double)getv(s, cb, k) + v));
 }
 void wv_solve_learned_start(const Ctx& s, wv_net* net, hipStream_t st) {
-  hipLaunchKernelGGL(k_net_scale, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
-  hipLaunchKernelGGL(k_net_scale, dim3(1), dim3(WV_RED), 0, st, s, 1);
+  hipLaunchKernelGGL(k_net_scale<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, 0);
+  hipLaunchKernelGGL(k_net_scale<true>, dim3(1), dim3(WV_RED), 0, st, s, 1);
   hipLaunchKernelGGL(k_net_features, dim3(nb(s.N, 256)), dim3(256), 0, st, s, wv_net_input_dev(net));
   wv_net_forward_dev(net, st);
   const float* qf = wv_net_qf_dev(net); const float* qc = wv_net_qc_dev(net);
-  hipLaunchKernelGGL(k_ws_dots, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, qf, qc, 0);
-  hipLaunchKernelGGL(k_ws_dots, dim3(1), dim3(WV_RED), 0, st, s, qf, qc, 1);
+  hipLaunchKernelGGL(k_ws_dots<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, qf, qc, 0);
+  hipLaunchKernelGGL(k_ws_dots<true>, dim3(1), dim3(WV_RED), 0, st, s, qf, qc, 1);
   hipLaunchKernelGGL(k_ws_update, dim3(nb(s.N, 256)), dim3(256), 0, st, s, qf, qc);
   hipLaunchKernelGGL(k_init_given, dim3(nb(s.N, 256)), dim3(256), 0, st, s);   // r = b - A p
 }
@@ -331,9 +337,10 @@ void wv_solve_learned_start(const Ctx& s, wv_net* net, hipStream_t st) {
 // Global liquid volume sum_i F_i h_x h_y h_z of one bank in FP64 (the FP32
 // volume of the acceptance op, pressure_pass.glslinc:78-103, is c[17]);
 // deterministic: fixed 128-lane groups, then one group -> dctl[13].
-__global__ void __launch_bounds__(WV_RED) k_volume64(Ctx s, int bank, int final) {
+template <bool FINAL>
+__global__ void __launch_bounds__(WV_RED) k_volume64(Ctx s, int bank, int) {
   int64_t N = s.N;
-  if (!final) {
+  if (!FINAL) {
     int64_t i = GID();
     d4 v = mkd(i < N ? (double)getv(s, bank, i) : 0.0, 0, 0, 0);
     v = group_reduce_d(v); put_partial(s, v); return;
@@ -342,6 +349,6 @@ __global__ void __launch_bounds__(WV_RED) k_volume64(Ctx s, int bank, int final)
   if (threadIdx.x == 0) s.dctl[13] = g.x * ((double)s.h[0] * (double)s.h[1] * (double)s.h[2]);
 }
 void wv_solve_volume64(const Ctx& s, int bank, hipStream_t st) {
-  hipLaunchKernelGGL(k_volume64, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, bank, 0);
-  hipLaunchKernelGGL(k_volume64, dim3(1), dim3(WV_RED), 0, st, s, bank, 1);
+  hipLaunchKernelGGL(k_volume64<false>, dim3(nb(s.N, WV_RED)), dim3(WV_RED), 0, st, s, bank, 0);
+  hipLaunchKernelGGL(k_volume64<true>, dim3(1), dim3(WV_RED), 0, st, s, bank, 1);
 }
