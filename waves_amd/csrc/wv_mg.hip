@@ -877,6 +877,21 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int 
     if (d.x == off.x && d.y == off.y && d.z == off.z) {
       const int E0 = F.box[0], E1 = F.box[1], E2 = F.box[2];
       const float* boxes = reinterpret_cast<const float*>(sbox);
+      // A node three or more nodes from the x and y edges and two from the
+      // periodic z seam (no periodic x), on a z axis longer than twice the
+      // box: every range below is one box position, or none when that
+      // position lies outside the box (the z wrap of a negative position
+      // lands past the box); the same terms in the same order.
+      if (!m.perx && qc.x >= 3 && qc.x <= nc.x - 4 && qc.y >= 3 && qc.z >= 2 && qc.z <= nc.z - 3 && nc.z >= 2 * E2 + 8) {
+        for (int c = 0; c < total; c++) {
+          const float w = wsup[c]; if (w == 0) continue;
+          const int bx = J[0] - asup[c][0], by = J[1] - asup[c][1], bz = J[2] - asup[c][2];
+          if ((unsigned)bx < (unsigned)E0 && (unsigned)by < (unsigned)E1 && (unsigned)bz < (unsigned)E2)
+            acc += w * boxes[c * Ep + bx + E0 * (by + E1 * bz)];
+        }
+        row[(int64_t)k * C.count] = acc;
+        return;
+      }
       for (int c = 0; c < total; c++) {
         const float w = wsup[c]; if (w == 0) continue;
         const int ax = asup[c][0], ay = asup[c][1], az = asup[c][2];
