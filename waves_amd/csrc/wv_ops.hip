@@ -210,8 +210,15 @@ __global__ void __launch_bounds__(256) k_distance_direct(Ctx s, int cap) {
 constexpr int DIST_THREADS = 64;  // threads per block of the queued distance pass
 __global__ void __launch_bounds__(DIST_THREADS) k_distance_deferred(Ctx s, int cap) {
   if (s.c[4] > .5f && s.c[5] < .5f) return;
-  __shared__ float verts[2 * VERT_FLOATS * DIST_THREADS];
-  const Verts<DIST_THREADS> va{verts + threadIdx.x}, vb{verts + VERT_FLOATS * DIST_THREADS + threadIdx.x};
+  // One 12-vertex list per thread.  polygon_distance fills up to 12 vertices
+  // (a plane through cube corners meets up to 12 edges) in va; the transition
+  // search clips a face square by two planes, va -> vb -> va, and a convex
+  // polygon clipped by a plane gains at most one vertex (4 -> 5 -> 6), so its
+  // two lists take the two halves of the same slot (the routines run one
+  // after the other, never together): half the LDS of two full lists, two
+  // more waves per CU.
+  __shared__ float verts[VERT_FLOATS * DIST_THREADS];
+  const Verts<DIST_THREADS> va{verts + threadIdx.x}, vb{verts + (VERT_FLOATS / 2) * DIST_THREADS + threadIdx.x};
   const QueueView qv = queue_view(s, 2, cap);
   const int64_t n = qv.total, groups = (int64_t)gridDim.x * blockDim.x / 32;
   const int sub = (int)(threadIdx.x & 31);
