@@ -777,23 +777,35 @@ __global__ void __launch_bounds__(GAL_THREADS) k_mg_galerkin_ptap(Ctx s, MGDev m
       // box entry b = bx + E0 (by + E1 bz); its coarse node's offset from I is
       // taken from the parent coordinates directly (mdelta of the encoded node)
       I3 nc = mdims(m, lc);
-      int b = 0;
-      float4 quad = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int bz = 0; bz < E2; bz++) for (int by = 0; by < E1; by++) for (int bx = 0; bx < E0; bx++, b++) {
-        if ((b & 3) == 0) quad = T[b >> 2];
-        float v = (b & 3) == 0 ? quad.x : (b & 3) == 1 ? quad.y : (b & 3) == 2 ? quad.z : quad.w;
-        if (v == 0) continue;
-        int y = anchor[1] + by;
-        if (y >= nc.y) { s.c[7] = 31; continue; }   // parent_node's dropped node: no slot
+      // box entry b = bx + E0 (by + E1 bz), x fastest, walked a quad at a
+      // time with the next quad's load in flight, so that the load latency
+      // overlaps the accumulator updates
+      int bx = 0, by = 0, bz = 0;
+      auto entry = [&](float v) {
+        const int ex = bx, ey = by, ez = bz;
+        if (++bx == E0) { bx = 0; if (++by == E1) { by = 0; ++bz; } }
+        if (v == 0) return;
+        int y = anchor[1] + ey;
+        if (y >= nc.y) { s.c[7] = 31; return; }   // parent_node's dropped node: no slot
         if (y < 0) y = 0;
-        int x = m.perx ? pmod(anchor[0] + bx, nc.x) : iclamp(anchor[0] + bx, 0, nc.x - 1);
-        int z = pmod(anchor[2] + bz, nc.z);
+        int x = m.perx ? pmod(anchor[0] + ex, nc.x) : iclamp(anchor[0] + ex, 0, nc.x - 1);
+        int z = pmod(anchor[2] + ez, nc.z);
         I3 d = mk(x - qc.x, y - qc.y, z - qc.z);
         if (m.perx) d.x = pmod(d.x + nc.x / 2, nc.x) - nc.x / 2;
         d.z = pmod(d.z + nc.z / 2, nc.z) - nc.z / 2;
         int sl = in_box2(d) ? slots[slot125(d)] : -1;
-        if (sl < 0) { s.c[7] = 31; continue; }
+        if (sl < 0) { s.c[7] = 31; return; }
         acc[sl * GAL_THREADS + tid] += wI * v;
+      };
+      const int q4 = (E + 3) >> 2;
+      float4 cur = T[0];
+      for (int q = 0; q < q4; q++) {
+        const float4 nxt = T[q + 1 < q4 ? q + 1 : q];
+        entry(cur.x);
+        if (4 * q + 1 < E) entry(cur.y);
+        if (4 * q + 2 < E) entry(cur.z);
+        if (4 * q + 3 < E) entry(cur.w);
+        cur = nxt;
       }
     }
   }
