@@ -675,50 +675,76 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_ap(Ctx s, MGDev m, int l, i
   if (live) {
   I3 q = coord(i, mdims(m, l)); int qa[3] = {q.x, q.y, q.z};
   int anchor[3]; for (int a = 0; a < 3; a++) anchor[a] = parent_base(F, a, qa[a] - F.margin[a]);
-  int width = F.width;
-  for (int k = lane; k < width; k += G) {
-    int off[3]; float aij; int j;
-    if (l == 0) {
-      off[0] = off[1] = off[2] = 0;
-      if (k == 0) { aij = snap(m, 12, i); j = (int)i; }
-      else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; off[axis] = 2 * side - 1; aij = -snap(m, 6 + f6, i);
-             j = mwrap(m, 0, mk(q.x + off[0], q.y + off[1], q.z + off[2])); }
-    } else {
-      I3 o = moffset(m, l, k); off[0] = o.x; off[1] = o.y; off[2] = o.z;
-      j = mwrap(m, l, mk(q.x + o.x, q.y + o.y, q.z + o.z));
-      const float a_row = m.rows[F.row_off + (int64_t)k * F.count + i];  // loaded before j is known
-      aij = j < 0 ? 0 : a_row;
-    }
-    // the neighbour's flag at an in-range index, loaded with the coefficient
-    // (mactive(m, l, j) for j >= 0)
-    const float fj = m.vec[at(m, l, 3, j >= 0 ? j : 0)];
-    if (j < 0 || aij == 0 || !(fj > .5f)) continue;
-    int ju[3] = {qa[0] + off[0], qa[1] + off[1], qa[2] + off[2]};  // unwrapped neighbour
-    int base[3]; float t[3];
-    for (int a = 0; a < 3; a++) {
-      if (F.aggregate || F.f[a] == 1) { base[a] = parent_base(F, a, ju[a]); t[a] = 0; }
-      else { float pos = lattice_pos(ju[a], F.f[a]); base[a] = (int)floorf(pos); t[a] = fract(pos); }
-    }
-    // at most 2 x 2 x 2 parents (coarsening factors are 1 or 2): their
-    // indices and weights first, then all their flags loaded together, then
-    // the accumulation in the same x-fastest order
-    const int nb0 = F.aggregate ? 1 : F.f[0], nb1 = F.aggregate ? 1 : F.f[1], nb2 = F.aggregate ? 1 : F.f[2];
-    int Jc[8], bc[8]; float wc[8], fc[8];
+  // bit b: the coarse node at box entry b = bx + E0 (by + E1 bz) exists and
+  // is active.  Every parent the row's entries reach lies in the box, so the
+  // flags are loaded once per row, eight at a time, not once per (entry,
+  // parent) pair.
+  uint64_t live_parents = 0;
+  {
+    int bx = 0, by = 0, bz = 0;
+    for (int b0 = 0; b0 < E; b0 += 8) {
+      int J[8]; float fl[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
-      int bit[3] = {x, y, z}; float w = 1; int P[3];
+      for (int u = 0; u < 8; u++) {
+        const int P[3] = {anchor[0] + bx, anchor[1] + by, anchor[2] + bz};
+        J[u] = b0 + u < E ? parent_node(m, lc, P) : -1;
+        fl[u] = m.vec[at(m, lc, 3, J[u] >= 0 ? J[u] : 0)];
+        if (++bx == E0) { bx = 0; if (++by == E1) { by = 0; ++bz; } }
+      }
 #pragma unroll
-      for (int a = 0; a < 3; a++) { P[a] = base[a] + bit[a]; if (!F.aggregate && F.f[a] > 1) w *= bit[a] == 1 ? t[a] : 1 - t[a]; }
-      const bool in = x < nb0 && y < nb1 && z < nb2;
-      const int J = in ? parent_node(m, lc, P) : -1;
-      Jc[c] = J; wc[c] = w; bc[c] = (P[0] - anchor[0]) + E0 * ((P[1] - anchor[1]) + E1 * (P[2] - anchor[2]));
-      fc[c] = m.vec[at(m, lc, 3, J >= 0 ? J : 0)];
+      for (int u = 0; u < 8; u++) if (J[u] >= 0 && fl[u] > .5f) live_parents |= 1ull << (b0 + u);
+    }
+  }
+  const int width = F.width;
+  // the row's entries k = lane, lane + G, ... eight at a time: their offsets,
+  // coefficients and neighbour flags loaded together, then accumulated in k
+  // order
+  for (int k0 = lane; k0 < width; k0 += 8 * G) {
+    int o8[8][3], j8[8]; float a8[8], f8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = k0 + u * G;
+      int off[3] = {0, 0, 0}; float aij = 0; int j = -1;
+      if (k < width) {
+        if (l == 0) {
+          if (k == 0) { aij = snap(m, 12, i); j = (int)i; }
+          else { int f6 = k - 1, axis = f6 >> 1, side = f6 & 1; off[axis] = 2 * side - 1; aij = -snap(m, 6 + f6, i);
+                 j = mwrap(m, 0, mk(q.x + off[0], q.y + off[1], q.z + off[2])); }
+        } else {
+          I3 o = moffset(m, l, k); off[0] = o.x; off[1] = o.y; off[2] = o.z;
+          j = mwrap(m, l, mk(q.x + o.x, q.y + o.y, q.z + o.z));
+          const float a_row = m.rows[F.row_off + (int64_t)k * F.count + i];  // loaded before j is known
+          aij = j < 0 ? 0 : a_row;
+        }
+      }
+      // the neighbour's flag at an in-range index (mactive(m, l, j) for j >= 0)
+      f8[u] = m.vec[at(m, l, 3, j >= 0 ? j : 0)];
+      o8[u][0] = off[0]; o8[u][1] = off[1]; o8[u][2] = off[2]; j8[u] = j; a8[u] = aij;
     }
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      if (Jc[c] < 0 || wc[c] == 0 || !(fc[c] > .5f)) continue;
-      acc[bc[c] * 128 + tid] += aij * wc[c];
+    for (int u = 0; u < 8; u++) {
+      const float aij = a8[u];
+      if (j8[u] < 0 || aij == 0 || !(f8[u] > .5f)) continue;
+      int ju[3] = {qa[0] + o8[u][0], qa[1] + o8[u][1], qa[2] + o8[u][2]};  // unwrapped neighbour
+      int base[3]; float t[3];
+      for (int a = 0; a < 3; a++) {
+        if (F.aggregate || F.f[a] == 1) { base[a] = parent_base(F, a, ju[a]); t[a] = 0; }
+        else { float pos = lattice_pos(ju[a], F.f[a]); base[a] = (int)floorf(pos); t[a] = fract(pos); }
+      }
+      // at most 2 x 2 x 2 parents (coarsening factors are 1 or 2), accumulated
+      // in x-fastest order; a parent's flag is its bit in live_parents
+      const int nb0 = F.aggregate ? 1 : F.f[0], nb1 = F.aggregate ? 1 : F.f[1], nb2 = F.aggregate ? 1 : F.f[2];
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
+        if (!(x < nb0 && y < nb1 && z < nb2)) continue;
+        int bit[3] = {x, y, z}; float w = 1;
+#pragma unroll
+        for (int a = 0; a < 3; a++) if (!F.aggregate && F.f[a] > 1) w *= bit[a] == 1 ? t[a] : 1 - t[a];
+        const int bc = (base[0] + x - anchor[0]) + E0 * ((base[1] + y - anchor[1]) + E1 * (base[2] + z - anchor[2]));
+        if (w == 0 || !((live_parents >> bc) & 1)) continue;
+        acc[bc * 128 + tid] += aij * w;
+      }
     }
   }
   }
