@@ -893,9 +893,11 @@ __global__ void __launch_bounds__(128) k_mg_galerkin_gather(Ctx s, MGDev m, int 
   }
   __syncthreads();
   // the support's (A P) boxes, read once with 16-byte loads (a box is at most
-  // 64 floats, contiguous and padded to 4); the entries are summed from LDS
+  // 64 floats, contiguous and padded to 4); the entries are summed from LDS.
+  // The launch sizes sbox to GATHER_SUPPORT boxes of this level (not of the
+  // largest box), which sets how many blocks share a CU.
   const int Ep = (F.box[0] * F.box[1] * F.box[2] + 3) & ~3, q4 = Ep >> 2;
-  __shared__ float4 sbox[GATHER_SUPPORT * 16];
+  extern __shared__ float4 sbox[];
   for (int e = k; e < total * q4; e += (int)blockDim.x) {
     const int c = e / q4, r = e - c * q4;
     sbox[c * q4 + r] = wsup[c] != 0 ? reinterpret_cast<const float4*>(m.ap + (int64_t)isup[c] * Ep)[r] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1070,7 +1072,8 @@ void wv_mg_build(const Ctx& s, const MGDev& m, hipStream_t st) {
     hipLaunchKernelGGL(k_mg_galerkin_ap, dim3(nblocks((int64_t)m.lv[l].count * ga, 128)), dim3(128), ap_lds, st, s, m, l, ga);
     // wide coarse stencils (125 entries: a block's 128 lanes) gather per entry;
     // narrow ones split each node's fine support over lanes with LDS slices
-    if (m.lv[l + 1].width > 64) hipLaunchKernelGGL(k_mg_galerkin_gather, dim3(m.lv[l + 1].count), dim3(128), 0, st, s, m, l);
+    const size_t gather_lds = sizeof(float4) * GATHER_SUPPORT * ((m.lv[l].box[0] * m.lv[l].box[1] * m.lv[l].box[2] + 3) / 4);
+    if (m.lv[l + 1].width > 64) hipLaunchKernelGGL(k_mg_galerkin_gather, dim3(m.lv[l + 1].count), dim3(128), gather_lds, st, s, m, l);
     else hipLaunchKernelGGL(k_mg_galerkin_ptap, dim3(nblocks((int64_t)m.lv[l + 1].count * G, GAL_THREADS)), dim3(GAL_THREADS), lds, st, s, m, l);
     hipLaunchKernelGGL(k_mg_symmetrize, dim3(nblocks((int64_t)m.lv[l + 1].count * SYM_LANES, 256)), dim3(256), 0, st, s, m, l + 1);
     hipLaunchKernelGGL(k_mg_l1_coarse, dim3(nblocks(m.lv[l + 1].count, 256)), dim3(256), 0, st, s, m, l + 1);
